@@ -1,0 +1,148 @@
+/*
+ * xss_gpu.h -- C ABI of the MI355X (gfx950) sort/select library.
+ *
+ * Device-pointer entry points.  Every function is stream-ordered and
+ * asynchronous: it enqueues kernels on `stream` (NULL = the default stream)
+ * and returns; results are valid once the stream has drained.  Scratch memory
+ * comes from a per-(device, stream) workspace the library keeps and grows.
+ * Return value: XSS_OK or an error code; xss_gpu_last_error() describes the
+ * last error of the calling thread.  Semantics (NaN placement, canonical NaN
+ * bits, ascending/descending, k handling) are those of the x86simdsort::
+ * functions each entry point replaces:
+ *
+ *   xss_gpu_qsort_<T>              x86simdsort::qsort<T>           lib/x86simdsort.h:15-18, lib/x86simdsort.cpp:60-66
+ *   xss_gpu_qselect_<T>            x86simdsort::qselect<T>         lib/x86simdsort.h:20-26, lib/x86simdsort.cpp:68-76
+ *   xss_gpu_partial_qsort_<T>      x86simdsort::partial_qsort<T>   lib/x86simdsort.h:28-34, lib/x86simdsort.cpp:78-87
+ *   xss_gpu_argsort_<T>            x86simdsort::argsort<T>         lib/x86simdsort.h:36-39, lib/x86simdsort.cpp:89-98
+ *   xss_gpu_argselect_<T>          x86simdsort::argselect<T>       lib/x86simdsort.h:41-44, lib/x86simdsort.cpp:100-109
+ *   xss_gpu_keyvalue_qsort_<K>_<V>        x86simdsort::keyvalue_qsort<K,V>        lib/x86simdsort.h:46-52, lib/x86simdsort.cpp:207-221
+ *   xss_gpu_keyvalue_select_<K>_<V>       x86simdsort::keyvalue_select<K,V>       lib/x86simdsort.h:54-61, lib/x86simdsort.cpp:222-233
+ *   xss_gpu_keyvalue_partial_sort_<K>_<V> x86simdsort::keyvalue_partial_sort<K,V> lib/x86simdsort.h:63-70, lib/x86simdsort.cpp:234-246
+ *
+ * <T> in {u16, i16, f16, u32, i32, f32, u64, i64, f64}; <K>,<V> in
+ * {u32, i32, f32, u64, i64, f64} (the 36 pairs of lib/x86simdsort.cpp:280-293).
+ * argsort/argselect write n size_t (uint64_t) indices to `arg` and leave the
+ * keys untouched, like the reference (src/x86simdsort-static-incl.h:92-121).
+ *
+ * Limits: n < 2^30 per array.  Host-pointer forms are the C++ templates of
+ * x86simdsort.h and the 8 keyvalue_qsort_* symbols declared at the end.
+ */
+#ifndef XSS_GPU_H
+#define XSS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define XSS_API __attribute__((visibility("default")))
+#else
+#define XSS_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *xss_stream_t; /* == hipStream_t */
+typedef uint16_t xss_f16_bits;             /* _Float16 passed as its bit pattern */
+
+enum {
+    XSS_OK = 0,
+    XSS_ERR_INVALID = 1,     /* bad argument (null pointer, k >= n, n too large) */
+    XSS_ERR_HIP = 2,         /* a HIP runtime call failed */
+    XSS_ERR_NOMEM = 3,       /* device scratch could not be allocated */
+    XSS_ERR_CAPTURE = 4,     /* workspace must grow while the stream is being captured */
+    XSS_ERR_NO_DEVICE = 5    /* no HIP device */
+};
+
+XSS_API const char *xss_gpu_last_error(void);
+XSS_API const char *xss_gpu_version(void);
+/* Release the calling thread's workspaces on every stream (synchronises). */
+XSS_API int xss_gpu_release_workspaces(void);
+/* Bytes of device scratch currently held by all workspaces. */
+XSS_API size_t xss_gpu_workspace_bytes(void);
+
+#define XSS_DECLARE_T(tag, T)                                                                      \
+    XSS_API int xss_gpu_qsort_##tag(T *arr, size_t n, int hasnan, int descending, xss_stream_t stream);    \
+    XSS_API int xss_gpu_qselect_##tag(T *arr, size_t k, size_t n, int hasnan, int descending,              \
+                              xss_stream_t stream);                                                \
+    XSS_API int xss_gpu_partial_qsort_##tag(T *arr, size_t k, size_t n, int hasnan, int descending,        \
+                                    xss_stream_t stream);                                          \
+    XSS_API int xss_gpu_argsort_##tag(const T *arr, uint64_t *arg, size_t n, int hasnan, int descending,   \
+                              xss_stream_t stream);                                                \
+    XSS_API int xss_gpu_argselect_##tag(const T *arr, uint64_t *arg, size_t k, size_t n, int hasnan,       \
+                                xss_stream_t stream);
+
+XSS_DECLARE_T(u16, uint16_t)
+XSS_DECLARE_T(i16, int16_t)
+XSS_DECLARE_T(f16, xss_f16_bits)
+XSS_DECLARE_T(u32, uint32_t)
+XSS_DECLARE_T(i32, int32_t)
+XSS_DECLARE_T(f32, float)
+XSS_DECLARE_T(u64, uint64_t)
+XSS_DECLARE_T(i64, int64_t)
+XSS_DECLARE_T(f64, double)
+
+#define XSS_DECLARE_KV(ktag, K, vtag, V)                                                           \
+    XSS_API int xss_gpu_keyvalue_qsort_##ktag##_##vtag(K *key, V *val, size_t n, int hasnan,               \
+                                               int descending, xss_stream_t stream);               \
+    XSS_API int xss_gpu_keyvalue_select_##ktag##_##vtag(K *key, V *val, size_t k, size_t n, int hasnan,    \
+                                                int descending, xss_stream_t stream);              \
+    XSS_API int xss_gpu_keyvalue_partial_sort_##ktag##_##vtag(K *key, V *val, size_t k, size_t n,          \
+                                                      int hasnan, int descending,                  \
+                                                      xss_stream_t stream);
+
+#define XSS_DECLARE_KV_ALLV(ktag, K)     \
+    XSS_DECLARE_KV(ktag, K, u32, uint32_t) \
+    XSS_DECLARE_KV(ktag, K, i32, int32_t)  \
+    XSS_DECLARE_KV(ktag, K, f32, float)    \
+    XSS_DECLARE_KV(ktag, K, u64, uint64_t) \
+    XSS_DECLARE_KV(ktag, K, i64, int64_t)  \
+    XSS_DECLARE_KV(ktag, K, f64, double)
+
+XSS_DECLARE_KV_ALLV(u32, uint32_t)
+XSS_DECLARE_KV_ALLV(i32, int32_t)
+XSS_DECLARE_KV_ALLV(f32, float)
+XSS_DECLARE_KV_ALLV(u64, uint64_t)
+XSS_DECLARE_KV_ALLV(i64, int64_t)
+XSS_DECLARE_KV_ALLV(f64, double)
+
+/*
+ * object_qsort support (lib/x86simdsort.h:72-106).  The reference computes
+ * one 32/64-bit key per object on the host, argsorts the keys with
+ * keyvalue_qsort(keys, iota) and permutes the objects in place along cycles.
+ * Here the permutation is a device gather: xss_gpu_object_sort_by_keys_<K>
+ * sorts (key, index) pairs ascending (hasnan = false, as the reference) and
+ * gathers `obj_bytes`-sized objects into that order, in place in `objs`.
+ * `keys` is overwritten with the sorted keys.
+ */
+XSS_API int xss_gpu_object_sort_by_keys_f64(void *objs, size_t obj_bytes, double *keys, size_t n,
+                                    xss_stream_t stream);
+XSS_API int xss_gpu_object_sort_by_keys_f32(void *objs, size_t obj_bytes, float *keys, size_t n,
+                                    xss_stream_t stream);
+XSS_API int xss_gpu_object_sort_by_keys_u64(void *objs, size_t obj_bytes, uint64_t *keys, size_t n,
+                                    xss_stream_t stream);
+XSS_API int xss_gpu_object_sort_by_keys_i64(void *objs, size_t obj_bytes, int64_t *keys, size_t n,
+                                    xss_stream_t stream);
+XSS_API int xss_gpu_object_sort_by_keys_u32(void *objs, size_t obj_bytes, uint32_t *keys, size_t n,
+                                    xss_stream_t stream);
+XSS_API int xss_gpu_object_sort_by_keys_i32(void *objs, size_t obj_bytes, int32_t *keys, size_t n,
+                                    xss_stream_t stream);
+
+/*
+ * Host-pointer C symbols of the reference (lib/x86simdsort.cpp:298-339):
+ * ascending key-value sort with hasnan = true, synchronous.
+ */
+XSS_API void keyvalue_qsort_float_uint32(float *key, uint32_t *val, size_t size);
+XSS_API void keyvalue_qsort_float_uint64(float *key, uint64_t *val, size_t size);
+XSS_API void keyvalue_qsort_uint64_uint32(uint64_t *key, uint32_t *val, size_t size);
+XSS_API void keyvalue_qsort_uint64_uint64(uint64_t *key, uint64_t *val, size_t size);
+XSS_API void keyvalue_qsort_int32_uint32(int32_t *key, uint32_t *val, size_t size);
+XSS_API void keyvalue_qsort_int32_uint64(int32_t *key, uint64_t *val, size_t size);
+XSS_API void keyvalue_qsort_uint32_uint32(uint32_t *key, uint32_t *val, size_t size);
+XSS_API void keyvalue_qsort_uint32_uint64(uint32_t *key, uint64_t *val, size_t size);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XSS_GPU_H */

@@ -1,0 +1,56 @@
+// xss_runtime.hpp -- device scratch workspaces and error reporting.
+//
+// The reference sorts in place on caller memory and allocates only the
+// argsort index vector (src/x86simdsort-static-incl.h:98-121).  A GPU radix
+// sort needs scratch: ping-pong key/payload buffers, digit histograms and
+// look-back words.  Each (device, stream) pair owns one workspace that grows
+// on demand and is reused by every later call on that stream, so steady-state
+// calls allocate nothing and work on one stream never races on scratch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <mutex>
+
+#include "xss_kernels.hpp"
+
+namespace xss {
+
+constexpr size_t CTRL_BYTES = 256;
+constexpr size_t OFFSETS_BYTES = 8 * 256 * sizeof(uint32_t);
+constexpr size_t PARTIAL_BYTES = size_t(HIST_MAX_BLOCKS) * 8 * 256 * sizeof(uint32_t);
+constexpr size_t META_FIXED = CTRL_BYTES + OFFSETS_BYTES + PARTIAL_BYTES;
+
+struct Workspace {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;             // held while a call enqueues work on this workspace
+    char *meta = nullptr;      // ctrl | offsets | partial | status
+    size_t meta_cap = 0;
+    char *data = nullptr;      // ping-pong scratch
+    size_t data_cap = 0;
+
+    Ctrl *ctrl() const { return reinterpret_cast<Ctrl *>(meta); }
+    uint32_t *offsets() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES); }
+    uint32_t *partial() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + OFFSETS_BYTES); }
+    uint64_t *status() const { return reinterpret_cast<uint64_t *>(meta + META_FIXED); }
+    size_t status_words() const { return (meta_cap - META_FIXED) / sizeof(uint64_t); }
+};
+
+// Returns the workspace of (current device, stream), locked, grown to hold
+// `status_words` look-back words and `data_bytes` of scratch.  On error
+// returns a status code and sets the thread's error text.
+int acquire_workspace(hipStream_t stream, size_t status_words, size_t data_bytes, Workspace **out,
+                      std::unique_lock<std::mutex> *lock);
+
+int set_error(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+}  // namespace xss
+
+#define XSS_HIP_CHECK(call)                                        \
+    do {                                                           \
+        hipError_t xss_e_ = (call);                                \
+        if (xss_e_ != hipSuccess) return ::xss::hip_fail(xss_e_, #call); \
+    } while (0)

@@ -1,0 +1,185 @@
+// xss_sort.hpp -- host-side pipeline of one radix sort on a stream.
+//
+// Replaces the driver layer of the reference (xss_qsort
+// src/xss-common-qsort.h:652-710, xss_qsort_kv
+// src/xss-common-keyvaluesort.hpp:564-653, xss_argsort
+// src/xss-common-argsort.h:587-669): instead of NaN pre/post passes, an
+// OpenMP team and a recursion, a sort is either one LDS block-sort launch
+// (small arrays) or hist -> scan -> P onesweep passes (P = key bits / 8).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/xss_gpu.h"
+#include "xss_kernels.hpp"
+#include "xss_runtime.hpp"
+
+namespace xss {
+
+constexpr uint32_t MAX_N = (1u << 30) - 1;
+
+template <class T>
+struct TypeKey;
+template <> struct TypeKey<uint16_t> { using KD = KeyDesc<16, Kind::Unsigned>; };
+template <> struct TypeKey<int16_t> { using KD = KeyDesc<16, Kind::Signed>; };
+struct f16_tag {};
+template <> struct TypeKey<f16_tag> { using KD = KeyDesc<16, Kind::Float>; };
+template <> struct TypeKey<uint32_t> { using KD = KeyDesc<32, Kind::Unsigned>; };
+template <> struct TypeKey<int32_t> { using KD = KeyDesc<32, Kind::Signed>; };
+template <> struct TypeKey<float> { using KD = KeyDesc<32, Kind::Float>; };
+template <> struct TypeKey<uint64_t> { using KD = KeyDesc<64, Kind::Unsigned>; };
+template <> struct TypeKey<int64_t> { using KD = KeyDesc<64, Kind::Signed>; };
+template <> struct TypeKey<double> { using KD = KeyDesc<64, Kind::Float>; };
+
+// payload mode of a value type: its width is all that matters
+template <class V>
+constexpr int payload_of() { return sizeof(V) == 4 ? PL_V32 : PL_V64; }
+
+// tile shape of a pass: BLOCK threads x ITEMS keys
+template <class KD, int PL>
+struct PassShape {
+    using U = typename KD::U;
+    using V = typename PayloadT<PL>::type;
+    static constexpr int BLOCK = 256;
+    static constexpr int ITEMS = (sizeof(U) + (PL == PL_NONE ? 0 : sizeof(V))) > 8 ? 8 : 16;
+    static constexpr int TILE = BLOCK * ITEMS;
+};
+
+// largest array one workgroup sorts in LDS (<= 64 KiB of LDS)
+template <class KD, int PL>
+struct BlockShape {
+    using U = typename KD::U;
+    using V = typename PayloadT<PL>::type;
+    static constexpr int BPE = 2 * sizeof(U) + 2 + ((PL == PL_V32 || PL == PL_V64) ? sizeof(V) : 0);
+    static constexpr int MAXN = (65536 / BPE) >= 4096 ? 4096 : (65536 / BPE) >= 2048 ? 2048 : 1024;
+    static constexpr int BLOCK = 512;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+template <class KD, int PL>
+struct SortPlan {
+    using U = typename KD::U;
+    using V = typename PayloadT<PL>::type;
+    static constexpr int P = KD::PASSES;
+    static bool small(size_t n) { return n <= size_t(BlockShape<KD, PL>::MAXN); }
+    static size_t tiles(size_t n) { return (n + PassShape<KD, PL>::TILE - 1) / PassShape<KD, PL>::TILE; }
+    static size_t status_words(size_t n) { return small(n) ? 0 : tiles(n) * 256; }
+    static size_t scratch_bytes(size_t n)
+    {
+        if (small(n)) return 0;
+        if (PL == PL_IDX) return (P > 2 ? 2 : 1) * (align256(n * sizeof(U)) + align256(n * 4));
+        if (PL == PL_NONE) return align256(n * sizeof(U));
+        return align256(n * sizeof(U)) + align256(n * sizeof(V));
+    }
+};
+
+// Enqueue one sort of n keys (in place when keys_in == keys_out).
+// `scratch` must hold SortPlan::scratch_bytes(n); the workspace's look-back
+// area must hold SortPlan::status_words(n).
+template <class KD, int PL>
+int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_out, const void *vals_in,
+                 void *vals_out, uint64_t *idx_out, size_t n, bool hasnan, bool desc, bool canon,
+                 hipStream_t s)
+{
+    using U = typename KD::U;
+    using V = typename PayloadT<PL>::type;
+    using Plan = SortPlan<KD, PL>;
+    const uint64_t flip = desc ? ~0ull : 0ull;
+    const bool nan_last = (KD::KIND == Kind::Float) && hasnan;
+    if (n < 2) {
+        // n <= 1 is a no-op in the reference (src/xss-common-qsort.h:660,
+        // src/xss-common-keyvaluesort.hpp:585); argsort still returns [0].
+        if (n == 1 && PL == PL_IDX) {
+            hipLaunchKernelGGL(fill_u64_kernel<0>, dim3(1), dim3(64), 0, s, idx_out, uint64_t(0), uint64_t(1));
+            XSS_HIP_CHECK(hipGetLastError());
+        }
+        return XSS_OK;
+    }
+    if (Plan::small(n)) {
+        using BS = BlockShape<KD, PL>;
+        BlockSortArgs b{};
+        b.keys_in = keys_in; b.keys_out = keys_out; b.vals_in = vals_in; b.vals_out = vals_out;
+        b.idx_out = idx_out; b.n = uint32_t(n); b.nan_last = nan_last; b.flip = flip;
+        b.canon_nan = canon && nan_last;
+        hipLaunchKernelGGL((block_sort_kernel<KD, PL, BS::BLOCK, BS::MAXN>), dim3(1), dim3(BS::BLOCK), 0, s, b);
+        XSS_HIP_CHECK(hipGetLastError());
+        return XSS_OK;
+    }
+    constexpr int P = KD::PASSES;
+    using PS = PassShape<KD, PL>;
+    const uint32_t nn = uint32_t(n);
+    const size_t tiles = Plan::tiles(n);
+
+    // histogram of all digit positions
+    constexpr int VEC = 16 / sizeof(U);
+    size_t hb = (n + size_t(HIST_BLOCK) * VEC * 8 - 1) / (size_t(HIST_BLOCK) * VEC * 8);
+    hb = std::max<size_t>(1, std::min<size_t>(hb, HIST_MAX_BLOCKS));
+    HistArgs h{};
+    h.keys = keys_in; h.n = nn; h.nan_last = nan_last; h.flip = flip; h.partial = ws->partial();
+    h.ctrl = ws->ctrl(); h.status = ws->status(); h.status_words = ws->status_words();
+    hipLaunchKernelGGL((hist_kernel<KD>), dim3(uint32_t(hb)), dim3(HIST_BLOCK), 0, s, h);
+    XSS_HIP_CHECK(hipGetLastError());
+    ScanArgs sc{};
+    sc.partial = ws->partial(); sc.nblocks = uint32_t(hb); sc.passes = P; sc.offsets = ws->offsets();
+    sc.ctrl = ws->ctrl();
+    hipLaunchKernelGGL(scan_kernel<0>, dim3(P), dim3(256), 0, s, sc);
+    XSS_HIP_CHECK(hipGetLastError());
+
+    // ping-pong buffers
+    char *k0 = scratch, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
+    if (PL == PL_IDX) {
+        v0 = k0 + align256(n * sizeof(U));
+        if (P > 2) {
+            k1 = v0 + align256(n * 4);
+            v1 = k1 + align256(n * sizeof(U));
+        }
+    } else if (PL != PL_NONE) {
+        v0 = k0 + align256(n * sizeof(U));
+    }
+    const void *src_k = keys_in, *src_v = vals_in;
+    for (int p = 0; p < P; ++p) {
+        PassArgs a{};
+        a.keys_in = src_k; a.vals_in = src_v; a.idx_out = idx_out; a.n = nn; a.shift = 8 * p;
+        a.flip = flip; a.nan_last = nan_last; a.offsets = ws->offsets() + p * 256;
+        a.status = ws->status(); a.ctrl = ws->ctrl(); a.pass = p;
+        a.first = (p == 0); a.last = (p == P - 1);
+        a.canon_nan = (p == P - 1) && canon && nan_last;
+        if (PL == PL_IDX) {
+            if (p < P - 1) {
+                a.keys_out = (p % 2 == 0) ? k0 : k1;
+                a.vals_out = (p % 2 == 0) ? v0 : v1;
+            }
+        } else {
+            a.keys_out = (p % 2 == 0) ? static_cast<void *>(k0) : keys_out;
+            a.vals_out = (p % 2 == 0) ? static_cast<void *>(v0) : vals_out;
+        }
+        hipLaunchKernelGGL((pass_kernel<KD, PL, PS::BLOCK, PS::ITEMS>), dim3(uint32_t(tiles)), dim3(PS::BLOCK), 0,
+                           s, a);
+        XSS_HIP_CHECK(hipGetLastError());
+        src_k = a.keys_out;
+        src_v = a.vals_out;
+    }
+    return XSS_OK;
+}
+
+// Whole-call wrappers: validate, acquire the stream's workspace, enqueue.
+template <class KD, int PL>
+int run_sort(const void *keys_in, void *keys_out, const void *vals_in, void *vals_out, uint64_t *idx_out,
+             size_t n, bool hasnan, bool desc, bool canon, hipStream_t s)
+{
+    if (n > MAX_N) return set_error(XSS_ERR_INVALID, "n = %zu exceeds the per-call limit %u", n, MAX_N);
+    if (n == 0) return XSS_OK;
+    if (!keys_in || (PL == PL_IDX ? !idx_out : !keys_out) || ((PL == PL_V32 || PL == PL_V64) && (!vals_in || !vals_out)))
+        return set_error(XSS_ERR_INVALID, "null pointer argument");
+    using Plan = SortPlan<KD, PL>;
+    Workspace *ws = nullptr;
+    std::unique_lock<std::mutex> lock;
+    int rc = acquire_workspace(s, Plan::status_words(n), Plan::scratch_bytes(n), &ws, &lock);
+    if (rc) return rc;
+    return enqueue_sort<KD, PL>(ws, ws->data, keys_in, keys_out, vals_in, vals_out, idx_out, n, hasnan, desc,
+                                canon, s);
+}
+
+}  // namespace xss

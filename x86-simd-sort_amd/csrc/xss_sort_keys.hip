@@ -1,0 +1,28 @@
+// xss_sort_keys.hip -- C-ABI entry points xss_gpu_qsort_<T> and
+// xss_gpu_argsort_<T> (x86simdsort::qsort / argsort, lib/x86simdsort.h:15-18,36-39).
+#include "xss_sort.hpp"
+
+using namespace xss;
+
+#define XSS_DEF_SORT(tag, T, CT)                                                                      \
+    extern "C" int xss_gpu_qsort_##tag(T *arr, size_t n, int hasnan, int descending, xss_stream_t s)  \
+    {                                                                                                 \
+        return run_sort<TypeKey<CT>::KD, PL_NONE>(arr, arr, nullptr, nullptr, nullptr, n, hasnan != 0, \
+                                                   descending != 0, true, s);                         \
+    }                                                                                                 \
+    extern "C" int xss_gpu_argsort_##tag(const T *arr, uint64_t *arg, size_t n, int hasnan,           \
+                                         int descending, xss_stream_t s)                              \
+    {                                                                                                 \
+        return run_sort<TypeKey<CT>::KD, PL_IDX>(arr, nullptr, nullptr, nullptr, arg, n, hasnan != 0,  \
+                                                  descending != 0, false, s);                         \
+    }
+
+XSS_DEF_SORT(u16, uint16_t, uint16_t)
+XSS_DEF_SORT(i16, int16_t, int16_t)
+XSS_DEF_SORT(f16, xss_f16_bits, f16_tag)
+XSS_DEF_SORT(u32, uint32_t, uint32_t)
+XSS_DEF_SORT(i32, int32_t, int32_t)
+XSS_DEF_SORT(f32, float, float)
+XSS_DEF_SORT(u64, uint64_t, uint64_t)
+XSS_DEF_SORT(i64, int64_t, int64_t)
+XSS_DEF_SORT(f64, double, double)

@@ -1,0 +1,145 @@
+"""ctypes binding of libxss_amd.so (the C ABI declared in include/xss_gpu.h).
+
+Host-side mirror of the x86simdsort:: API for Python callers, tests and the
+benchmark.  Arrays live on the GPU as torch tensors; every call enqueues on
+torch's current stream and is asynchronous like the C entry points.  The
+functions keep the reference's names and argument meaning
+(lib/x86simdsort.h:15-70): qsort, qselect, partial_qsort, argsort, argselect,
+keyvalue_qsort, keyvalue_select, keyvalue_partial_sort, object_qsort.
+
+There is no CPU fallback: if the library is missing or a call fails, a
+RuntimeError is raised.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libxss_amd.so")
+
+# type tag <-> numpy dtype (_Float16 is numpy float16)
+TAGS = {
+    "u16": np.uint16, "i16": np.int16, "f16": np.float16,
+    "u32": np.uint32, "i32": np.int32, "f32": np.float32,
+    "u64": np.uint64, "i64": np.int64, "f64": np.float64,
+}
+KV_TAGS = ["u32", "i32", "f32", "u64", "i64", "f64"]
+TAG_OF = {np.dtype(v): k for k, v in TAGS.items()}
+
+XSS_OK = 0
+
+_lib = None
+
+
+def lib():
+    """Load libxss_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run x86-simd-sort_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        L.xss_gpu_last_error.restype = ctypes.c_char_p
+        L.xss_gpu_version.restype = ctypes.c_char_p
+        L.xss_gpu_workspace_bytes.restype = ctypes.c_size_t
+        _lib = L
+    return _lib
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _check(rc, what):
+    if rc != XSS_OK:
+        msg = lib().xss_gpu_last_error().decode()
+        raise RuntimeError(f"{what} failed (status {rc}): {msg}")
+
+
+class DArray:
+    """A typed 1-D array in GPU memory (raw bytes in a torch tensor)."""
+
+    def __init__(self, tag, n, tensor):
+        self.tag, self.n, self.t = tag, n, tensor
+
+    @classmethod
+    def from_numpy(cls, a):
+        import torch
+        a = np.ascontiguousarray(a)
+        tag = TAG_OF[a.dtype]
+        t = torch.from_numpy(a.view(np.uint8).reshape(-1).copy()).cuda()
+        return cls(tag, a.size, t)
+
+    @classmethod
+    def empty(cls, tag, n):
+        import torch
+        item = np.dtype(TAGS[tag]).itemsize
+        return cls(tag, n, torch.empty(max(1, n * item), dtype=torch.uint8, device="cuda"))
+
+    @property
+    def ptr(self):
+        return ctypes.c_void_p(self.t.data_ptr())
+
+    def numpy(self):
+        import torch
+        torch.cuda.current_stream().synchronize()
+        item = np.dtype(TAGS[self.tag]).itemsize
+        return self.t[: self.n * item].cpu().numpy().view(TAGS[self.tag]).copy()
+
+
+def _sz(x):
+    return ctypes.c_size_t(int(x))
+
+
+def qsort(a: DArray, hasnan=False, descending=False):
+    _check(getattr(lib(), f"xss_gpu_qsort_{a.tag}")(a.ptr, _sz(a.n), int(hasnan), int(descending), _stream()),
+           f"qsort<{a.tag}>")
+
+
+def qselect(a: DArray, k, hasnan=False, descending=False):
+    _check(getattr(lib(), f"xss_gpu_qselect_{a.tag}")(a.ptr, _sz(k), _sz(a.n), int(hasnan), int(descending),
+                                                      _stream()), f"qselect<{a.tag}>")
+
+
+def partial_qsort(a: DArray, k, hasnan=False, descending=False):
+    _check(getattr(lib(), f"xss_gpu_partial_qsort_{a.tag}")(a.ptr, _sz(k), _sz(a.n), int(hasnan), int(descending),
+                                                            _stream()), f"partial_qsort<{a.tag}>")
+
+
+def argsort(a: DArray, hasnan=False, descending=False, out: DArray = None):
+    out = out or DArray.empty("u64", a.n)
+    _check(getattr(lib(), f"xss_gpu_argsort_{a.tag}")(a.ptr, out.ptr, _sz(a.n), int(hasnan), int(descending),
+                                                      _stream()), f"argsort<{a.tag}>")
+    return out
+
+
+def argselect(a: DArray, k, hasnan=False, out: DArray = None):
+    out = out or DArray.empty("u64", a.n)
+    _check(getattr(lib(), f"xss_gpu_argselect_{a.tag}")(a.ptr, out.ptr, _sz(k), _sz(a.n), int(hasnan), _stream()),
+           f"argselect<{a.tag}>")
+    return out
+
+
+def keyvalue_qsort(key: DArray, val: DArray, hasnan=False, descending=False):
+    _check(getattr(lib(), f"xss_gpu_keyvalue_qsort_{key.tag}_{val.tag}")(
+        key.ptr, val.ptr, _sz(key.n), int(hasnan), int(descending), _stream()), f"keyvalue_qsort<{key.tag},{val.tag}>")
+
+
+def keyvalue_select(key: DArray, val: DArray, k, hasnan=False, descending=False):
+    _check(getattr(lib(), f"xss_gpu_keyvalue_select_{key.tag}_{val.tag}")(
+        key.ptr, val.ptr, _sz(k), _sz(key.n), int(hasnan), int(descending), _stream()),
+        f"keyvalue_select<{key.tag},{val.tag}>")
+
+
+def keyvalue_partial_sort(key: DArray, val: DArray, k, hasnan=False, descending=False):
+    _check(getattr(lib(), f"xss_gpu_keyvalue_partial_sort_{key.tag}_{val.tag}")(
+        key.ptr, val.ptr, _sz(k), _sz(key.n), int(hasnan), int(descending), _stream()),
+        f"keyvalue_partial_sort<{key.tag},{val.tag}>")
+
+
+def object_sort_by_keys(objs, obj_bytes, keys: DArray):
+    """Sort `objs` (a CUDA uint8 tensor of n*obj_bytes) by `keys` ascending."""
+    _check(getattr(lib(), f"xss_gpu_object_sort_by_keys_{keys.tag}")(
+        ctypes.c_void_p(objs.data_ptr()), _sz(obj_bytes), keys.ptr, _sz(keys.n), _stream()),
+        f"object_sort_by_keys<{keys.tag}>")

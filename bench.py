@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Sort throughput of the HIP radix sort on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--dtype f32|u32] [--n N]
+
+A step is one x86simdsort::qsort (device-pointer form, xss_gpu_qsort_<T>) of
+N = 2^28 keys already resident in HBM: uniform over the full bit range of the
+type (finite for floats), no NaN, hasnan = false, ascending.  The sort is in
+place, so the input is restored from a pristine copy before every step; like
+the reference's benchmark (benchmarks/bench-qsort.hpp:12-16, PauseTiming)
+the restore is outside the timed sort: each step is bracketed by HIP events
+on the stream the library launches on, and the JSON reports their sum.
+
+The JSON line carries:
+  value/ms_per_step   whole-sort throughput (Gkeys/s) and time per sort
+  roofline            the dominant kernel (one onesweep scatter pass): its
+                      algorithmic bytes per launch (N * 2 * key bytes: every
+                      key read once and written once) over its average launch
+                      time from the library's per-kernel HIP events, against
+                      MI355X's 8 TB/s; `traffic` is the measured HBM bytes per
+                      launch from profiles/ when a rocprofv3 --pmc summary is
+                      present there
+  pipeline            the whole sort against the roofline at
+                      (1 + 2P) * key bytes per key (histogram read + P passes)
+  cpu_baseline        the reference library (oracle/_ref, AVX-512 dispatch +
+                      OpenMP, <= 16 threads) timed on this host on a bounded
+                      sample of the same distribution
+"""
+import argparse
+import ctypes
+import json
+import os
+import statistics
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "x86-simd-sort_amd"))
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gkeys/s + % HBM roofline at 256M keys; segmented-batch vs per-call speedup"
+
+
+def device_input(torch, dtype, n, seed):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    x = torch.randint(-2 ** 31, 2 ** 31, (n,), dtype=torch.int32, device="cuda", generator=g)
+    if dtype == "f32":
+        # keep floats finite: exponent all-ones -> flip one exponent bit
+        bad = (x & 0x7F800000) == 0x7F800000
+        x = torch.where(bad, x ^ 0x40000000, x)
+    return x
+
+
+def lib_timings(L):
+    names = (ctypes.c_char_p * 32)()
+    ms = (ctypes.c_double * 32)()
+    cnt = (ctypes.c_uint64 * 32)()
+    k = L.xss_gpu_timing_collect(32, names, ms, cnt)
+    return {names[i].decode(): (ms[i], cnt[i]) for i in range(max(0, k))}
+
+
+def cpu_baseline(dtype, seconds=12.0):
+    """The reference itself (kind "reference") when oracle/_ref is built,
+    else the oracle's C restatement (kind "port", one thread)."""
+    import numpy as np
+    import xss_testlib as T
+    n = 1 << 25
+    a = T.gen(dtype, "random", n, 99)
+    L = T.ref()
+    if L is not None:
+        threads = min(16, os.cpu_count() or 1)
+        L.xssref_set_threads(threads)
+        fn, kind, isa = getattr(L, f"xssref_qsort_{dtype}"), "reference", L.xssref_isa().decode()
+    else:
+        L = T.oracle()
+        threads = 1
+        fn, kind, isa = getattr(L, f"xss_oracle_qsort_{dtype}"), "port", "scalar C"
+    times = []
+    t_end = time.time() + seconds
+    while len(times) < 3 or (time.time() < t_end and len(times) < 50):
+        b = a.copy()
+        t0 = time.perf_counter()
+        fn(ctypes.c_void_p(b.ctypes.data), ctypes.c_size_t(n), 0, 0)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": round(n / med / 1e9, 4), "unit": "Gkeys/s", "cores": threads, "kind": kind,
+            "sample": f"qsort<{dtype}> of 2^25 uniform full-range keys, median of {len(times)} sorts "
+                      f"(input restore excluded); dispatch {isa}"}
+
+
+def traffic_from_profiles(dtype, n):
+    p = os.path.join(REPO, "profiles", f"pass_kernel_{dtype}_256M_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        if int(d.get("n", 0)) != n:
+            return None
+        return d.get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--dtype", default="f32", choices=["f32", "u32"])
+    ap.add_argument("--n", type=int, default=1 << 28)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one MI355X, and the sort of one array "
+              f"does not span GPUs (--gpus {args.gpus} not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import xss_amd
+    L = xss_amd.lib()
+    torch.cuda.set_device(0)
+    n, dt = args.n, args.dtype
+    key_bytes = 4
+    pristine = device_input(torch, dt, n, 1234)
+    work = torch.empty_like(pristine)
+    arr = xss_amd.DArray(dt, n, work.view(torch.uint8))
+
+    def restore():
+        work.copy_(pristine)
+
+    for _ in range(args.warmup):
+        restore()
+        xss_amd.qsort(arr)
+    torch.cuda.synchronize()
+    lib_timings(L)
+
+    L.xss_gpu_timing_enable(1)
+    stream = torch.cuda.current_stream()
+    evs = []
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for _ in range(args.steps):
+        restore()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        xss_amd.qsort(arr)
+        e1.record(stream)
+        evs.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    L.xss_gpu_timing_enable(0)
+    kt = lib_timings(L)
+    sort_ms = [a.elapsed_time(b) for a, b in evs]
+    ms_per_step = sum(sort_ms) / len(sort_ms)
+
+    # correctness of the last step (ascending order of the float/uint values)
+    out = work if dt == "f32" else work
+    if dt == "f32":
+        v = out.view(torch.float32)
+        ok = bool((v[1:] >= v[:-1]).all().item())
+    else:
+        v = out.to(torch.int64) & 0xFFFFFFFF
+        ok = bool((v[1:] >= v[:-1]).all().item())
+
+    pass_ms, pass_cnt = kt.get("pass", (0.0, 0))
+    pass_avg_ms = pass_ms / max(1, pass_cnt)
+    pass_bytes = n * 2 * key_bytes
+    achieved = pass_bytes / (pass_avg_ms / 1e3) / 1e9 if pass_avg_ms > 0 else 0.0
+    P = 4
+    pipe_bytes = n * (1 + 2 * P) * key_bytes
+    pipe_gbs = pipe_bytes / (ms_per_step / 1e3) / 1e9
+    gkeys = n / (ms_per_step / 1e3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(gkeys, 3),
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if dt == "f32" else "u32",
+        "data": "synthetic (uniform full-range bits, finite, seeded on device)",
+        "config": {"workload": f"qsort<{'float' if dt == 'f32' else 'uint32_t'}> of 2^28 keys in HBM, "
+                               f"ascending, hasnan=false", "n": n, "key_bytes": key_bytes,
+                   "digit_bits": 8, "passes": P, "restore_excluded": True},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dt, n),
+                     "kernel": "pass_kernel (onesweep scatter pass)", "avg_launch_ms": round(pass_avg_ms, 4),
+                     "bytes_per_launch_alg": pass_bytes},
+        "pipeline": {"bytes_per_sort_alg": pipe_bytes, "achieved_GBs": round(pipe_gbs, 1),
+                     "frac": round(pipe_gbs / HBM_PEAK_GBS, 4),
+                     "kernels_ms_per_sort": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}},
+        "wall_s_timed_loop_incl_restore": round(wall, 4),
+        "sorted_ok": ok,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(dt)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

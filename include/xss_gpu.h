@@ -61,6 +61,12 @@ XSS_API const char *xss_gpu_version(void);
 XSS_API int xss_gpu_release_workspaces(void);
 /* Bytes of device scratch currently held by all workspaces. */
 XSS_API size_t xss_gpu_workspace_bytes(void);
+/* Per-kernel timing for profiling: when enabled, HIP events are recorded on
+ * the launch stream around every kernel the library enqueues.  collect()
+ * waits for them, returns up to max_names (name, total ms, launches) rows
+ * and clears the record.  No counterpart in the reference. */
+XSS_API int xss_gpu_timing_enable(int on);
+XSS_API int xss_gpu_timing_collect(int max_names, const char **names, double *total_ms, uint64_t *counts);
 
 #define XSS_DECLARE_T(tag, T)                                                                      \
     XSS_API int xss_gpu_qsort_##tag(T *arr, size_t n, int hasnan, int descending, xss_stream_t stream);    \

@@ -288,6 +288,22 @@ def assert_sorted_equal(got, exp, tag, what="", zeros_as_values=False):
     assert np.array_equal(np.sort(bg), np.sort(be)), f"{what}: bit patterns differ as multisets (+-0 / NaN runs)"
 
 
+def assert_prefix_equal(got, exp, k, tag, what=""):
+    """The first k of a partial sort.  A run of equal values cut by position k
+    (a +-0 run, a NaN run) may hold any of its members before k, so that one
+    run is compared by value; everything else bit-exactly."""
+    cg, ce = vclass(got[:k], tag), vclass(exp[:k], tag)
+    if not np.array_equal(cg, ce):
+        i = int(np.nonzero(cg != ce)[0][0])
+        raise AssertionError(f"{what}: first mismatch at {i}: got {got[max(0,i-2):i+3]} "
+                             f"expected {exp[max(0,i-2):i+3]}")
+    keep = np.ones(k, bool)
+    if 0 < k < exp.size and vclass(exp[k:k + 1], tag)[0] == ce[-1]:
+        keep = ce != ce[-1]
+    assert np.array_equal(np.sort(bits(got[:k])[keep]), np.sort(bits(exp[:k])[keep])), \
+        f"{what}: bit patterns of the sorted prefix differ"
+
+
 def assert_kv_equal(gk, gv, ek, ev, tag, what=""):
     assert_sorted_equal(gk, ek, tag, what + " keys")
     ce = vclass(ek, tag)

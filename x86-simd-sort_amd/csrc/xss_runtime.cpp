@@ -8,6 +8,7 @@
 #include <memory>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "../../include/xss_gpu.h"
 
@@ -98,9 +99,84 @@ int acquire_workspace(hipStream_t stream, size_t status_words, size_t data_bytes
     return XSS_OK;
 }
 
+namespace {
+struct TimedLaunch {
+    std::string name;
+    hipEvent_t a, b;
+};
+std::mutex g_t_mu;
+bool g_timing = false;
+std::vector<TimedLaunch> g_launches;
+std::vector<hipEvent_t> g_event_pool;
+hipEvent_t pooled_event()
+{
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+}  // namespace
+
+bool timing_on() { return g_timing; }
+
+void timing_begin(const char *name, hipStream_t s)
+{
+    std::lock_guard<std::mutex> g(g_t_mu);
+    TimedLaunch t{name, pooled_event(), pooled_event()};
+    (void)hipEventRecord(t.a, s);
+    g_launches.push_back(t);
+}
+
+void timing_end(hipStream_t s)
+{
+    std::lock_guard<std::mutex> g(g_t_mu);
+    if (!g_launches.empty()) (void)hipEventRecord(g_launches.back().b, s);
+}
+
 }  // namespace xss
 
 extern "C" {
+
+int xss_gpu_timing_enable(int on)
+{
+    std::lock_guard<std::mutex> g(xss::g_t_mu);
+    xss::g_timing = on != 0;
+    return XSS_OK;
+}
+
+int xss_gpu_timing_collect(int max_names, const char **names, double *total_ms, uint64_t *counts)
+{
+    std::lock_guard<std::mutex> g(xss::g_t_mu);
+    static std::vector<std::string> keep;
+    std::map<std::string, std::pair<double, uint64_t>> agg;
+    for (auto &t : xss::g_launches) {
+        XSS_HIP_CHECK(hipEventSynchronize(t.b));
+        float ms = 0.f;
+        XSS_HIP_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
+        auto &slot = agg[t.name];
+        slot.first += ms;
+        slot.second += 1;
+        xss::g_event_pool.push_back(t.a);
+        xss::g_event_pool.push_back(t.b);
+    }
+    xss::g_launches.clear();
+    keep.clear();
+    int i = 0;
+    for (auto &kv : agg) {
+        if (i >= max_names) break;
+        keep.push_back(kv.first);
+        names[i] = nullptr;
+        total_ms[i] = kv.second.first;
+        counts[i] = kv.second.second;
+        ++i;
+    }
+    for (int j = 0; j < i; ++j) names[j] = keep[j].c_str();
+    return i;
+}
 
 const char *xss_gpu_last_error(void) { return xss::g_err.c_str(); }
 

@@ -45,6 +45,18 @@ int acquire_workspace(hipStream_t stream, size_t status_words, size_t data_bytes
                       std::unique_lock<std::mutex> *lock);
 
 int set_error(int code, const char *fmt, ...);
+
+// Optional per-kernel timing (xss_gpu_timing_enable): HIP events recorded
+// on the launch stream around each kernel, summed per kernel name.
+bool timing_on();
+void timing_begin(const char *name, hipStream_t s);
+void timing_end(hipStream_t s);
+struct KernelTimer {
+    hipStream_t s;
+    bool on;
+    KernelTimer(const char *name, hipStream_t st) : s(st), on(timing_on()) { if (on) timing_begin(name, s); }
+    ~KernelTimer() { if (on) timing_end(s); }
+};
 int hip_fail(hipError_t e, const char *what);
 
 }  // namespace xss

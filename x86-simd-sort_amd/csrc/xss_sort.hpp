@@ -103,7 +103,10 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
         b.keys_in = keys_in; b.keys_out = keys_out; b.vals_in = vals_in; b.vals_out = vals_out;
         b.idx_out = idx_out; b.n = uint32_t(n); b.nan_last = nan_last; b.flip = flip;
         b.canon_nan = canon && nan_last;
-        hipLaunchKernelGGL((block_sort_kernel<KD, PL, BS::BLOCK, BS::MAXN>), dim3(1), dim3(BS::BLOCK), 0, s, b);
+        {
+            KernelTimer t("block_sort", s);
+            hipLaunchKernelGGL((block_sort_kernel<KD, PL, BS::BLOCK, BS::MAXN>), dim3(1), dim3(BS::BLOCK), 0, s, b);
+        }
         XSS_HIP_CHECK(hipGetLastError());
         return XSS_OK;
     }
@@ -119,12 +122,18 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
     HistArgs h{};
     h.keys = keys_in; h.n = nn; h.nan_last = nan_last; h.flip = flip; h.partial = ws->partial();
     h.ctrl = ws->ctrl(); h.status = ws->status(); h.status_words = ws->status_words();
-    hipLaunchKernelGGL((hist_kernel<KD>), dim3(uint32_t(hb)), dim3(HIST_BLOCK), 0, s, h);
+    {
+        KernelTimer t("hist", s);
+        hipLaunchKernelGGL((hist_kernel<KD>), dim3(uint32_t(hb)), dim3(HIST_BLOCK), 0, s, h);
+    }
     XSS_HIP_CHECK(hipGetLastError());
     ScanArgs sc{};
     sc.partial = ws->partial(); sc.nblocks = uint32_t(hb); sc.passes = P; sc.offsets = ws->offsets();
     sc.ctrl = ws->ctrl();
-    hipLaunchKernelGGL(scan_kernel<0>, dim3(P), dim3(256), 0, s, sc);
+    {
+        KernelTimer t("scan", s);
+        hipLaunchKernelGGL(scan_kernel<0>, dim3(P), dim3(256), 0, s, sc);
+    }
     XSS_HIP_CHECK(hipGetLastError());
 
     // ping-pong buffers
@@ -155,8 +164,11 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
             a.keys_out = (p % 2 == 0) ? static_cast<void *>(k0) : keys_out;
             a.vals_out = (p % 2 == 0) ? static_cast<void *>(v0) : vals_out;
         }
-        hipLaunchKernelGGL((pass_kernel<KD, PL, PS::BLOCK, PS::ITEMS>), dim3(uint32_t(tiles)), dim3(PS::BLOCK), 0,
-                           s, a);
+        {
+            KernelTimer t("pass", s);
+            hipLaunchKernelGGL((pass_kernel<KD, PL, PS::BLOCK, PS::ITEMS>), dim3(uint32_t(tiles)), dim3(PS::BLOCK), 0,
+                               s, a);
+        }
         XSS_HIP_CHECK(hipGetLastError());
         src_k = a.keys_out;
         src_v = a.vals_out;

@@ -13,15 +13,17 @@ on the stream the library launches on, and the JSON reports their sum.
 
 The JSON line carries:
   value/ms_per_step   whole-sort throughput (Gkeys/s) and time per sort
-  roofline            the dominant kernel (one onesweep scatter pass): its
+  roofline            the dominant kernel (one LSD scatter pass): its
                       algorithmic bytes per launch (N * 2 * key bytes: every
                       key read once and written once) over its average launch
                       time from the library's per-kernel HIP events, against
                       MI355X's 8 TB/s; `traffic` is the measured HBM bytes per
                       launch from profiles/ when a rocprofv3 --pmc summary is
                       present there
-  pipeline            the whole sort against the roofline at
-                      (1 + 2P) * key bytes per key (histogram read + P passes)
+  pipeline            the whole sort against the roofline at the algorithmic
+                      (1 + 2P) * key bytes per key of an 8-bit LSD sort; this
+                      design moves 3P * key bytes per key (each pass = one
+                      upsweep read + one downsweep read and write)
   cpu_baseline        the reference library (oracle/_ref, AVX-512 dispatch +
                       OpenMP, <= 16 threads) timed on this host on a bounded
                       sample of the same distribution
@@ -91,7 +93,7 @@ def cpu_baseline(dtype, seconds=12.0):
 
 
 def traffic_from_profiles(dtype, n):
-    p = os.path.join(REPO, "profiles", f"pass_kernel_{dtype}_256M_traffic.json")
+    p = os.path.join(REPO, "profiles", f"downsweep_{dtype}_256M_traffic.json")
     if not os.path.exists(p):
         return None
     try:
@@ -164,12 +166,15 @@ def main():
         v = out.to(torch.int64) & 0xFFFFFFFF
         ok = bool((v[1:] >= v[:-1]).all().item())
 
-    pass_ms, pass_cnt = kt.get("pass", (0.0, 0))
+    pass_ms, pass_cnt = kt.get("downsweep", (0.0, 0))
     pass_avg_ms = pass_ms / max(1, pass_cnt)
     pass_bytes = n * 2 * key_bytes
     achieved = pass_bytes / (pass_avg_ms / 1e3) / 1e9 if pass_avg_ms > 0 else 0.0
     P = 4
+    # algorithmic minimum of an 8-bit LSD sort: one histogram read + P x (read + write)
     pipe_bytes = n * (1 + 2 * P) * key_bytes
+    # what this design moves: P x (upsweep read + downsweep read + write)
+    moved_bytes = n * 3 * P * key_bytes
     pipe_gbs = pipe_bytes / (ms_per_step / 1e3) / 1e9
     gkeys = n / (ms_per_step / 1e3) / 1e9
     line = {
@@ -190,9 +195,10 @@ def main():
                    "digit_bits": 8, "passes": P, "restore_excluded": True},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dt, n),
-                     "kernel": "pass_kernel (onesweep scatter pass)", "avg_launch_ms": round(pass_avg_ms, 4),
+                     "kernel": "downsweep_kernel (one LSD scatter pass)", "avg_launch_ms": round(pass_avg_ms, 4),
                      "bytes_per_launch_alg": pass_bytes},
-        "pipeline": {"bytes_per_sort_alg": pipe_bytes, "achieved_GBs": round(pipe_gbs, 1),
+        "pipeline": {"bytes_per_sort_alg": pipe_bytes, "bytes_per_sort_moved": moved_bytes,
+                     "achieved_GBs": round(pipe_gbs, 1),
                      "frac": round(pipe_gbs / HBM_PEAK_GBS, 4),
                      "kernels_ms_per_sort": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}},
         "wall_s_timed_loop_incl_restore": round(wall, 4),

@@ -289,17 +289,17 @@ def assert_sorted_equal(got, exp, tag, what="", zeros_as_values=False):
 
 
 def assert_prefix_equal(got, exp, k, tag, what=""):
-    """The first k of a partial sort.  A run of equal values cut by position k
-    (a +-0 run, a NaN run) may hold any of its members before k, so that one
-    run is compared by value; everything else bit-exactly."""
+    """The first k of a partial sort.  The run of values equal to the k-th
+    element (a +-0 run, a NaN run with several payloads) may hold any of its
+    members before k and at k-1, so that run is compared by value; everything
+    else bit-exactly."""
     cg, ce = vclass(got[:k], tag), vclass(exp[:k], tag)
     if not np.array_equal(cg, ce):
         i = int(np.nonzero(cg != ce)[0][0])
         raise AssertionError(f"{what}: first mismatch at {i}: got {got[max(0,i-2):i+3]} "
                              f"expected {exp[max(0,i-2):i+3]}")
-    keep = np.ones(k, bool)
-    if 0 < k < exp.size and vclass(exp[k:k + 1], tag)[0] == ce[-1]:
-        keep = ce != ce[-1]
+    # the run holding the k-th element may be cut anywhere: compare it by value
+    keep = ce != ce[-1] if k > 0 else np.ones(0, bool)
     assert np.array_equal(np.sort(bits(got[:k])[keep]), np.sort(bits(exp[:k])[keep])), \
         f"{what}: bit patterns of the sorted prefix differ"
 

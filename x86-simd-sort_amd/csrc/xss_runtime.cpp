@@ -40,8 +40,7 @@ std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
 size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 }  // namespace
 
-int acquire_workspace(hipStream_t stream, size_t status_words, size_t data_bytes, Workspace **out,
-                      std::unique_lock<std::mutex> *lock)
+int acquire_workspace(hipStream_t stream, size_t data_bytes, Workspace **out, std::unique_lock<std::mutex> *lock)
 {
     int dev = 0;
     XSS_HIP_CHECK(hipGetDevice(&dev));
@@ -57,30 +56,20 @@ int acquire_workspace(hipStream_t stream, size_t status_words, size_t data_bytes
         ws = slot.get();
     }
     std::unique_lock<std::mutex> l(ws->mu);
-    const size_t meta_need = META_FIXED + status_words * sizeof(uint64_t);
-    const bool grow_meta = meta_need > ws->meta_cap;
+    const bool need_meta = ws->meta == nullptr;
     const bool grow_data = data_bytes > ws->data_cap;
-    if (grow_meta || grow_data) {
+    if (need_meta || grow_data) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         XSS_HIP_CHECK(hipStreamIsCapturing(stream, &cs));
         if (cs != hipStreamCaptureStatusNone)
             return set_error(XSS_ERR_CAPTURE,
-                             "workspace must grow (meta %zu, data %zu bytes) during stream capture; "
+                             "workspace must grow to %zu scratch bytes during stream capture; "
                              "run the call once before capturing",
-                             meta_need, data_bytes);
-        // earlier work on this stream may still use the old buffers
-        XSS_HIP_CHECK(hipStreamSynchronize(stream));
-        if (grow_meta) {
-            if (ws->meta) XSS_HIP_CHECK(hipFree(ws->meta));
-            ws->meta = nullptr;
-            ws->meta_cap = 0;
-            size_t cap = round_up(META_FIXED + (status_words + status_words / 4) * sizeof(uint64_t), 1 << 16);
-            XSS_HIP_CHECK(hipMalloc(&ws->meta, cap));
-            ws->meta_cap = cap;
-            // epoch 0 and all-zero look-back words: no stale word matches a live epoch
-            XSS_HIP_CHECK(hipMemsetAsync(ws->meta, 0, cap, stream));
-        }
+                             data_bytes);
+        if (need_meta) XSS_HIP_CHECK(hipMalloc(&ws->meta, META_BYTES));
         if (grow_data) {
+            // earlier work on this stream may still use the old buffer
+            XSS_HIP_CHECK(hipStreamSynchronize(stream));
             if (ws->data) XSS_HIP_CHECK(hipFree(ws->data));
             ws->data = nullptr;
             ws->data_cap = 0;
@@ -88,11 +77,6 @@ int acquire_workspace(hipStream_t stream, size_t status_words, size_t data_bytes
             XSS_HIP_CHECK(hipMalloc(&ws->data, cap));
             ws->data_cap = cap;
         }
-    } else if (!ws->meta) {
-        size_t cap = round_up(META_FIXED + 4096 * sizeof(uint64_t), 1 << 16);
-        XSS_HIP_CHECK(hipMalloc(&ws->meta, cap));
-        ws->meta_cap = cap;
-        XSS_HIP_CHECK(hipMemsetAsync(ws->meta, 0, cap, stream));
     }
     *out = ws;
     *lock = std::move(l);
@@ -195,7 +179,7 @@ int xss_gpu_release_workspaces(void)
         if (ws->meta) XSS_HIP_CHECK(hipFree(ws->meta));
         if (ws->data) XSS_HIP_CHECK(hipFree(ws->data));
         ws->meta = ws->data = nullptr;
-        ws->meta_cap = ws->data_cap = 0;
+        ws->data_cap = 0;
         XSS_HIP_CHECK(hipSetDevice(cur));
     }
     xss::g_ws.clear();
@@ -206,7 +190,7 @@ size_t xss_gpu_workspace_bytes(void)
 {
     std::lock_guard<std::mutex> g(xss::g_map_mu);
     size_t s = 0;
-    for (auto &kv : xss::g_ws) s += kv.second->meta_cap + kv.second->data_cap;
+    for (auto &kv : xss::g_ws) s += (kv.second->meta ? xss::META_BYTES : 0) + kv.second->data_cap;
     return s;
 }
 

@@ -18,31 +18,25 @@
 namespace xss {
 
 constexpr size_t CTRL_BYTES = 256;
-constexpr size_t OFFSETS_BYTES = 8 * 256 * sizeof(uint32_t);
-constexpr size_t PARTIAL_BYTES = size_t(HIST_MAX_BLOCKS) * 8 * 256 * sizeof(uint32_t);
-constexpr size_t META_FIXED = CTRL_BYTES + OFFSETS_BYTES + PARTIAL_BYTES;
+constexpr size_t RSUM_BYTES = size_t(MAX_RANGES) * 256 * sizeof(uint32_t);
+constexpr size_t META_BYTES = CTRL_BYTES + 2 * RSUM_BYTES;
 
 struct Workspace {
     int device = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;             // held while a call enqueues work on this workspace
-    char *meta = nullptr;      // ctrl | offsets | partial | status
-    size_t meta_cap = 0;
+    char *meta = nullptr;      // ctrl (select state) | rsum | rbase
     char *data = nullptr;      // ping-pong scratch
     size_t data_cap = 0;
 
-    Ctrl *ctrl() const { return reinterpret_cast<Ctrl *>(meta); }
-    uint32_t *offsets() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES); }
-    uint32_t *partial() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + OFFSETS_BYTES); }
-    uint64_t *status() const { return reinterpret_cast<uint64_t *>(meta + META_FIXED); }
-    size_t status_words() const { return (meta_cap - META_FIXED) / sizeof(uint64_t); }
+    uint32_t *rsum() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES); }
+    uint32_t *rbase() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + RSUM_BYTES); }
 };
 
-// Returns the workspace of (current device, stream), locked, grown to hold
-// `status_words` look-back words and `data_bytes` of scratch.  On error
-// returns a status code and sets the thread's error text.
-int acquire_workspace(hipStream_t stream, size_t status_words, size_t data_bytes, Workspace **out,
-                      std::unique_lock<std::mutex> *lock);
+// Returns the workspace of (current device, stream), locked, with at least
+// `data_bytes` of scratch.  On error returns a status code and sets the
+// thread's error text.
+int acquire_workspace(hipStream_t stream, size_t data_bytes, Workspace **out, std::unique_lock<std::mutex> *lock);
 
 int set_error(int code, const char *fmt, ...);
 

@@ -261,7 +261,7 @@ struct SelPlan {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
     static constexpr bool HAS_VAL = PL == PL_V32 || PL == PL_V64;
-    static uint32_t nblocks(size_t n) { return uint32_t(std::min<size_t>(HIST_MAX_BLOCKS, (n + 4095) / 4096)); }
+    static uint32_t nblocks(size_t n) { return uint32_t(std::min<size_t>(MAX_RANGES, (n + 4095) / 4096)); }
     static uint32_t chunk(size_t n)
     {
         const uint32_t nb = nblocks(n);
@@ -270,7 +270,7 @@ struct SelPlan {
     // scratch: copy of keys (+values) for in-place forms, candidates, counts
     static size_t scratch_bytes(size_t n, bool in_place)
     {
-        size_t s = align256(size_t(nblocks(n)) * chunk(n) * sizeof(U)) + align256(HIST_MAX_BLOCKS * 4);
+        size_t s = align256(size_t(nblocks(n)) * chunk(n) * sizeof(U)) + align256(MAX_RANGES * 4);
         if (in_place) s += align256(n * sizeof(U)) + (HAS_VAL ? align256(n * sizeof(V)) : 0);
         return s;
     }
@@ -295,12 +295,12 @@ int enqueue_select(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     a.nan_last = (KD::KIND == Kind::Float) && hasnan;
     a.flip = desc ? ~0ull : 0ull;
     a.st = sel_state(ws);
-    a.partial = ws->partial();
+    a.partial = ws->rsum();
     char *p = scratch;
     a.cand = p;
     p += align256(size_t(a.nblocks) * a.chunk * sizeof(U));
     a.ccount = reinterpret_cast<uint32_t *>(p);
-    p += align256(HIST_MAX_BLOCKS * 4);
+    p += align256(MAX_RANGES * 4);
     char *tmp_k = nullptr, *tmp_v = nullptr;
     if (in_place) {
         tmp_k = p;
@@ -365,14 +365,10 @@ int run_select(int kind, void *keys, void *vals, uint64_t *idx_out, size_t n, si
     }
     const bool in_place = PL != PL_IDX;
     size_t sb = SelPlan<KD, SPL>::scratch_bytes(n, in_place);
-    size_t sw = 0;
-    if (kind == 1) {
-        sb = std::max(sb, SortPlan<KD, SPL>::scratch_bytes(kk));
-        sw = SortPlan<KD, SPL>::status_words(kk);
-    }
+    if (kind == 1) sb = std::max(sb, SortPlan<KD, SPL>::scratch_bytes(kk));
     Workspace *ws = nullptr;
     std::unique_lock<std::mutex> lock;
-    int rc = acquire_workspace(s, sw, sb, &ws, &lock);
+    int rc = acquire_workspace(s, sb, &ws, &lock);
     if (rc) return rc;
     if (SortPlan<KD, PL>::small(n)) {
         // partial sort of a small array: sort it whole with qsort semantics

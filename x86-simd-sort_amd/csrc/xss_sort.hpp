@@ -36,7 +36,7 @@ template <> struct TypeKey<double> { using KD = KeyDesc<64, Kind::Float>; };
 template <class V>
 constexpr int payload_of() { return sizeof(V) == 4 ? PL_V32 : PL_V64; }
 
-// tile shape of a pass: BLOCK threads x ITEMS keys
+// tile shape of a downsweep: BLOCK threads x ITEMS keys
 template <class KD, int PL>
 struct PassShape {
     using U = typename KD::U;
@@ -63,9 +63,12 @@ struct SortPlan {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
     static constexpr int P = KD::PASSES;
+    static constexpr int TILE = PassShape<KD, PL>::TILE;
     static bool small(size_t n) { return n <= size_t(BlockShape<KD, PL>::MAXN); }
-    static size_t tiles(size_t n) { return (n + PassShape<KD, PL>::TILE - 1) / PassShape<KD, PL>::TILE; }
-    static size_t status_words(size_t n) { return small(n) ? 0 : tiles(n) * 256; }
+    static size_t tiles(size_t n) { return (n + TILE - 1) / TILE; }
+    // persistent workgroups: contiguous ranges of whole tiles
+    static uint32_t range_tiles(size_t n) { return uint32_t((tiles(n) + MAX_RANGES - 1) / MAX_RANGES); }
+    static uint32_t ranges(size_t n) { return uint32_t((tiles(n) + range_tiles(n) - 1) / range_tiles(n)); }
     static size_t scratch_bytes(size_t n)
     {
         if (small(n)) return 0;
@@ -76,15 +79,13 @@ struct SortPlan {
 };
 
 // Enqueue one sort of n keys (in place when keys_in == keys_out).
-// `scratch` must hold SortPlan::scratch_bytes(n); the workspace's look-back
-// area must hold SortPlan::status_words(n).
+// `scratch` must hold SortPlan::scratch_bytes(n).
 template <class KD, int PL>
 int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_out, const void *vals_in,
                  void *vals_out, uint64_t *idx_out, size_t n, bool hasnan, bool desc, bool canon,
                  hipStream_t s)
 {
     using U = typename KD::U;
-    using V = typename PayloadT<PL>::type;
     using Plan = SortPlan<KD, PL>;
     const uint64_t flip = desc ? ~0ull : 0ull;
     const bool nan_last = (KD::KIND == Kind::Float) && hasnan;
@@ -113,28 +114,7 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
     constexpr int P = KD::PASSES;
     using PS = PassShape<KD, PL>;
     const uint32_t nn = uint32_t(n);
-    const size_t tiles = Plan::tiles(n);
-
-    // histogram of all digit positions
-    constexpr int VEC = 16 / sizeof(U);
-    size_t hb = (n + size_t(HIST_BLOCK) * VEC * 8 - 1) / (size_t(HIST_BLOCK) * VEC * 8);
-    hb = std::max<size_t>(1, std::min<size_t>(hb, HIST_MAX_BLOCKS));
-    HistArgs h{};
-    h.keys = keys_in; h.n = nn; h.nan_last = nan_last; h.flip = flip; h.partial = ws->partial();
-    h.ctrl = ws->ctrl(); h.status = ws->status(); h.status_words = ws->status_words();
-    {
-        KernelTimer t("hist", s);
-        hipLaunchKernelGGL((hist_kernel<KD>), dim3(uint32_t(hb)), dim3(HIST_BLOCK), 0, s, h);
-    }
-    XSS_HIP_CHECK(hipGetLastError());
-    ScanArgs sc{};
-    sc.partial = ws->partial(); sc.nblocks = uint32_t(hb); sc.passes = P; sc.offsets = ws->offsets();
-    sc.ctrl = ws->ctrl();
-    {
-        KernelTimer t("scan", s);
-        hipLaunchKernelGGL(scan_kernel<0>, dim3(P), dim3(256), 0, s, sc);
-    }
-    XSS_HIP_CHECK(hipGetLastError());
+    const uint32_t rtiles = Plan::range_tiles(n), ranges = Plan::ranges(n);
 
     // ping-pong buffers
     char *k0 = scratch, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
@@ -149,10 +129,24 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
     }
     const void *src_k = keys_in, *src_v = vals_in;
     for (int p = 0; p < P; ++p) {
+        UpArgs u{};
+        u.keys = src_k; u.n = nn; u.range_keys = rtiles * PS::TILE; u.shift = 8 * p; u.nan_last = nan_last;
+        u.flip = flip; u.rsum = ws->rsum();
+        {
+            KernelTimer t("upsweep", s);
+            hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
+        }
+        XSS_HIP_CHECK(hipGetLastError());
+        ScanArgs sc{};
+        sc.rsum = ws->rsum(); sc.rbase = ws->rbase(); sc.ranges = ranges;
+        {
+            KernelTimer t("range_scan", s);
+            hipLaunchKernelGGL(range_scan_kernel<0>, dim3(1), dim3(SCAN_BLOCK), 0, s, sc);
+        }
+        XSS_HIP_CHECK(hipGetLastError());
         PassArgs a{};
         a.keys_in = src_k; a.vals_in = src_v; a.idx_out = idx_out; a.n = nn; a.shift = 8 * p;
-        a.flip = flip; a.nan_last = nan_last; a.offsets = ws->offsets() + p * 256;
-        a.status = ws->status(); a.ctrl = ws->ctrl(); a.pass = p;
+        a.flip = flip; a.nan_last = nan_last; a.range_tiles = rtiles; a.rbase = ws->rbase();
         a.first = (p == 0); a.last = (p == P - 1);
         a.canon_nan = (p == P - 1) && canon && nan_last;
         if (PL == PL_IDX) {
@@ -165,8 +159,8 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
             a.vals_out = (p % 2 == 0) ? static_cast<void *>(v0) : vals_out;
         }
         {
-            KernelTimer t("pass", s);
-            hipLaunchKernelGGL((pass_kernel<KD, PL, PS::BLOCK, PS::ITEMS>), dim3(uint32_t(tiles)), dim3(PS::BLOCK), 0,
+            KernelTimer t("downsweep", s);
+            hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS>), dim3(ranges), dim3(PS::BLOCK), 0,
                                s, a);
         }
         XSS_HIP_CHECK(hipGetLastError());
@@ -188,7 +182,7 @@ int run_sort(const void *keys_in, void *keys_out, const void *vals_in, void *val
     using Plan = SortPlan<KD, PL>;
     Workspace *ws = nullptr;
     std::unique_lock<std::mutex> lock;
-    int rc = acquire_workspace(s, Plan::status_words(n), Plan::scratch_bytes(n), &ws, &lock);
+    int rc = acquire_workspace(s, Plan::scratch_bytes(n), &ws, &lock);
     if (rc) return rc;
     return enqueue_sort<KD, PL>(ws, ws->data, keys_in, keys_out, vals_in, vals_out, idx_out, n, hasnan, desc,
                                 canon, s);

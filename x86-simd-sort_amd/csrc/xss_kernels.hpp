@@ -50,9 +50,9 @@ struct PayloadT { using type = uint32_t; };
 template <>
 struct PayloadT<PL_V64> { using type = uint64_t; };
 
-constexpr int MAX_RANGES = 1024;   // persistent workgroups per pass
+constexpr int MAX_RANGES = 4096;   // persistent workgroups per pass
 constexpr int UP_BLOCK = 256;
-constexpr int SCAN_BLOCK = 1024;
+constexpr int SCAN_BLOCK = 256;
 
 // Peer mask of the lanes holding the same 8-bit digit: per bit b, the ballot
 // B_b of bit b, and every lane keeps the lanes that agree with it,
@@ -63,8 +63,10 @@ __device__ __forceinline__ uint64_t match_any8(uint32_t d)
     uint32_t lo = ~0u, hi = ~0u;
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-        const uint32_t x = uint32_t(int32_t(d << (31 - b)) >> 31);
-        const uint64_t bal = __ballot(x != 0);
+        // x: bit b replicated (v_bfe_i32); the ballot compares x itself so the
+        // compiler does not re-derive the bit with a second shift
+        const uint32_t x = uint32_t(__builtin_amdgcn_sbfe(int32_t(d), b, 1));
+        const uint64_t bal = __builtin_amdgcn_uicmp(x, 0u, 33 /* ne */);
         lo = __builtin_amdgcn_bitop3_b32(lo, uint32_t(bal), x, 0x90);
         hi = __builtin_amdgcn_bitop3_b32(hi, uint32_t(bal >> 32), x, 0x90);
     }
@@ -91,7 +93,8 @@ struct UpArgs {
     uint32_t shift;
     uint32_t nan_last;
     uint64_t flip;
-    uint32_t *rsum;        // [ranges][256]
+    uint32_t *rsum;        // [256][ranges] (digit-major)
+    uint32_t ranges;
 };
 
 template <class KD>
@@ -107,7 +110,6 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
     const int shift = int(a.shift);
     const uint32_t r0 = blockIdx.x * a.range_keys;
     const uint32_t r1 = min(a.n, r0 + a.range_keys);
-    const uint32_t lane = threadIdx.x & 63;
     constexpr int VEC = 16 / sizeof(U);
     // keys before the first 16-byte boundary are read one by one
     const uint32_t mis = uint32_t(reinterpret_cast<uintptr_t>(keys + r0) & 15u);
@@ -122,9 +124,10 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
         for (int j = 0; j < VEC; ++j) {
             const uint32_t d = KD::digit(e[j], flip, nl, shift);
             // whole-wave runs of one digit (sorted or constant data) add once
+            const uint64_t act = __ballot(1);
             const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
             if (__ballot(d != d0) == 0) {
-                if (lane == 0) atomicAdd(&h[d0], uint32_t(__popcll(__ballot(1))));
+                if (lanes_below(act) == 0) atomicAdd(&h[d0], uint32_t(__popcll(act)));
             } else {
                 atomicAdd(&h[d], 1u);
             }
@@ -137,54 +140,71 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
         atomicAdd(&h[KD::digit(keys[i], flip, nl, shift)], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < 256; i += UP_BLOCK) a.rsum[size_t(blockIdx.x) * 256 + i] = h[i];
+    for (int i = threadIdx.x; i < 256; i += UP_BLOCK) a.rsum[size_t(i) * a.ranges + blockIdx.x] = h[i];
 }
 
 // ---------------------------------------------------------------- range scan
 
 struct ScanArgs {
-    const uint32_t *rsum;  // [ranges][256]
-    uint32_t *rbase;       // [ranges][256] global start of each (range, digit) slice
+    const uint32_t *rsum;  // [256][ranges]
+    uint32_t *rpre;        // [256][ranges] exclusive prefix over ranges, per digit
+    uint32_t *total;       // [256] keys per digit
     uint32_t ranges;
 };
 
-// one workgroup of 1024 threads: 4 groups x 256 digits
+// workgroup d scans digit d's column (ranges <= 16 * SCAN_BLOCK)
 template <int = 0>
 __global__ __launch_bounds__(SCAN_BLOCK) void range_scan_kernel(ScanArgs a)
 {
-    __shared__ uint32_t part[4][256];
-    __shared__ uint32_t wsum[4];
-    const uint32_t d = threadIdx.x & 255, g = threadIdx.x >> 8;
-    const uint32_t per = (a.ranges + 3) / 4;
-    const uint32_t b0 = g * per, b1 = min(a.ranges, b0 + per);
-    uint32_t run = 0;
-    for (uint32_t b = b0; b < b1; ++b) {
-        const uint32_t c = a.rsum[size_t(b) * 256 + d];
-        a.rbase[size_t(b) * 256 + d] = run;
-        run += c;
+    constexpr int PER = MAX_RANGES / SCAN_BLOCK;
+    __shared__ uint32_t wsum[SCAN_BLOCK / 64];
+    const uint32_t d = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t *col = a.rsum + size_t(d) * a.ranges;
+    uint32_t *out = a.rpre + size_t(d) * a.ranges;
+    uint32_t v[PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const uint32_t b = t * PER + i;
+        v[i] = b < a.ranges ? col[b] : 0u;
+        sum += v[i];
     }
-    part[g][d] = run;
-    __syncthreads();
-    // group prefixes and per-digit totals
-    uint32_t gpre = 0, tot = 0;
-    for (uint32_t i = 0; i < 4; ++i) {
-        if (i < g) gpre += part[i][d];
-        tot += part[i][d];
-    }
-    // exclusive scan of the totals over the 256 digits (threads of group 0..)
-    uint32_t incl = tot;
-    const uint32_t lane = threadIdx.x & 63, w = (threadIdx.x >> 6) & 3;
+    uint32_t incl = sum;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t t = __shfl_up(incl, off);
-        if (lane >= uint32_t(off)) incl += t;
+        const uint32_t x = __shfl_up(incl, off);
+        if (lane >= uint32_t(off)) incl += x;
     }
-    if (g == 0 && lane == 63) wsum[w] = incl;
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    for (uint32_t i = 0; i < w; ++i) run += wsum[i];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const uint32_t b = t * PER + i;
+        if (b < a.ranges) out[b] = run;
+        run += v[i];
+    }
+    if (t == SCAN_BLOCK - 1) a.total[d] = run;
+}
+
+// exclusive scan of the 256 digit totals by the first 256 threads of a
+// workgroup (all threads must call it): returns the global start of digit tid
+__device__ __forceinline__ uint32_t digit_starts(const uint32_t *total, uint32_t *wsum4)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t tot = tid < 256 ? total[tid] : 0u;
+    uint32_t incl = tot;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t x = __shfl_up(incl, off);
+        if (lane >= uint32_t(off)) incl += x;
+    }
+    if (lane == 63 && w < 4) wsum4[w] = incl;
     __syncthreads();
     uint32_t pre = 0;
-    for (uint32_t i = 0; i < w; ++i) pre += wsum[i];
-    const uint32_t goff = pre + incl - tot;
-    for (uint32_t b = b0; b < b1; ++b) a.rbase[size_t(b) * 256 + d] += goff + gpre;
+    for (uint32_t i = 0; i < w && i < 4; ++i) pre += wsum4[i];
+    return pre + incl - tot;
 }
 
 // ---------------------------------------------------------------- downsweep
@@ -203,11 +223,13 @@ struct PassArgs {
     uint32_t last;           // PL_IDX: write size_t indices only
     uint32_t canon_nan;      // qsort(hasnan): write canonical NaN bits
     uint32_t range_tiles;    // tiles per workgroup
-    const uint32_t *rbase;   // [ranges][256]
+    uint32_t ranges;
+    const uint32_t *rpre;    // [256][ranges]
+    const uint32_t *total;   // [256]
 };
 
-template <class KD, int PL, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void downsweep_kernel(PassArgs a)
+template <class KD, int PL, int BLOCK, int ITEMS, int MINW = 1>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW, 8))) void downsweep_kernel(PassArgs a)
 {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
@@ -221,7 +243,7 @@ __global__ __launch_bounds__(BLOCK) void downsweep_kernel(PassArgs a)
     __shared__ uint32_t s_cnt[WAVES * 256];
     __shared__ uint32_t s_start[256];
     __shared__ uint32_t s_base[256];
-    __shared__ uint32_t s_wsum[WAVES];
+    __shared__ uint32_t s_wsum[WAVES < 4 ? 4 : WAVES];
     __shared__ U s_keys[TILE];
     __shared__ V s_vals[HAS_PL ? TILE : 1];
 
@@ -237,7 +259,8 @@ __global__ __launch_bounds__(BLOCK) void downsweep_kernel(PassArgs a)
     const uint32_t tile0 = blockIdx.x * a.range_tiles;
     const uint32_t tile1 = min(ntiles, tile0 + a.range_tiles);
     // running global base of each digit of this range (thread d owns digit d)
-    uint32_t run = tid < 256 ? a.rbase[size_t(blockIdx.x) * 256 + tid] : 0u;
+    const uint32_t dstart = digit_starts(a.total, s_wsum);
+    uint32_t run = tid < 256 ? dstart + a.rpre[size_t(tid) * a.ranges + blockIdx.x] : 0u;
 
     for (uint32_t tile = tile0; tile < tile1; ++tile) {
         for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) s_cnt[i] = 0;

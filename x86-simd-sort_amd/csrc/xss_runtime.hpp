@@ -19,18 +19,19 @@ namespace xss {
 
 constexpr size_t CTRL_BYTES = 256;
 constexpr size_t RSUM_BYTES = size_t(MAX_RANGES) * 256 * sizeof(uint32_t);
-constexpr size_t META_BYTES = CTRL_BYTES + 2 * RSUM_BYTES;
+constexpr size_t META_BYTES = CTRL_BYTES + 2 * RSUM_BYTES + 1024;
 
 struct Workspace {
     int device = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;             // held while a call enqueues work on this workspace
-    char *meta = nullptr;      // ctrl (select state) | rsum | rbase
+    char *meta = nullptr;      // ctrl (select state) | rsum | rpre | digit totals
     char *data = nullptr;      // ping-pong scratch
     size_t data_cap = 0;
 
     uint32_t *rsum() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES); }
-    uint32_t *rbase() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + RSUM_BYTES); }
+    uint32_t *rpre() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + RSUM_BYTES); }
+    uint32_t *total() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + 2 * RSUM_BYTES); }
 };
 
 // Returns the workspace of (current device, stream), locked, with at least

@@ -32,6 +32,7 @@ namespace xss {
 
 constexpr int SEL_BLOCK = 256;
 constexpr int SEL_TILE = SEL_BLOCK * 16;
+constexpr int SEL_MAX_BLOCKS = 1024;  // chunked select kernels; partial rows live in the rsum area
 
 struct SelState {
     uint64_t prefix;    // ord bits of K fixed so far
@@ -261,7 +262,7 @@ struct SelPlan {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
     static constexpr bool HAS_VAL = PL == PL_V32 || PL == PL_V64;
-    static uint32_t nblocks(size_t n) { return uint32_t(std::min<size_t>(MAX_RANGES, (n + 4095) / 4096)); }
+    static uint32_t nblocks(size_t n) { return uint32_t(std::min<size_t>(SEL_MAX_BLOCKS, (n + 4095) / 4096)); }
     static uint32_t chunk(size_t n)
     {
         const uint32_t nb = nblocks(n);
@@ -270,7 +271,7 @@ struct SelPlan {
     // scratch: copy of keys (+values) for in-place forms, candidates, counts
     static size_t scratch_bytes(size_t n, bool in_place)
     {
-        size_t s = align256(size_t(nblocks(n)) * chunk(n) * sizeof(U)) + align256(MAX_RANGES * 4);
+        size_t s = align256(size_t(nblocks(n)) * chunk(n) * sizeof(U)) + align256(SEL_MAX_BLOCKS * 4);
         if (in_place) s += align256(n * sizeof(U)) + (HAS_VAL ? align256(n * sizeof(V)) : 0);
         return s;
     }
@@ -300,7 +301,7 @@ int enqueue_select(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     a.cand = p;
     p += align256(size_t(a.nblocks) * a.chunk * sizeof(U));
     a.ccount = reinterpret_cast<uint32_t *>(p);
-    p += align256(MAX_RANGES * 4);
+    p += align256(SEL_MAX_BLOCKS * 4);
     char *tmp_k = nullptr, *tmp_v = nullptr;
     if (in_place) {
         tmp_k = p;

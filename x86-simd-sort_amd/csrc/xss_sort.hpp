@@ -131,22 +131,23 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
     for (int p = 0; p < P; ++p) {
         UpArgs u{};
         u.keys = src_k; u.n = nn; u.range_keys = rtiles * PS::TILE; u.shift = 8 * p; u.nan_last = nan_last;
-        u.flip = flip; u.rsum = ws->rsum();
+        u.flip = flip; u.rsum = ws->rsum(); u.ranges = ranges;
         {
             KernelTimer t("upsweep", s);
             hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
         }
         XSS_HIP_CHECK(hipGetLastError());
         ScanArgs sc{};
-        sc.rsum = ws->rsum(); sc.rbase = ws->rbase(); sc.ranges = ranges;
+        sc.rsum = ws->rsum(); sc.rpre = ws->rpre(); sc.total = ws->total(); sc.ranges = ranges;
         {
             KernelTimer t("range_scan", s);
-            hipLaunchKernelGGL(range_scan_kernel<0>, dim3(1), dim3(SCAN_BLOCK), 0, s, sc);
+            hipLaunchKernelGGL(range_scan_kernel<0>, dim3(256), dim3(SCAN_BLOCK), 0, s, sc);
         }
         XSS_HIP_CHECK(hipGetLastError());
         PassArgs a{};
         a.keys_in = src_k; a.vals_in = src_v; a.idx_out = idx_out; a.n = nn; a.shift = 8 * p;
-        a.flip = flip; a.nan_last = nan_last; a.range_tiles = rtiles; a.rbase = ws->rbase();
+        a.flip = flip; a.nan_last = nan_last; a.range_tiles = rtiles; a.ranges = ranges;
+        a.rpre = ws->rpre(); a.total = ws->total();
         a.first = (p == 0); a.last = (p == P - 1);
         a.canon_nan = (p == P - 1) && canon && nan_last;
         if (PL == PL_IDX) {

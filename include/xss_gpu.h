@@ -118,22 +118,29 @@ XSS_DECLARE_KV_ALLV(f64, double)
  * one 32/64-bit key per object on the host, argsorts the keys with
  * keyvalue_qsort(keys, iota) and permutes the objects in place along cycles.
  * Here the permutation is a device gather: xss_gpu_object_sort_by_keys_<K>
- * sorts (key, index) pairs ascending (hasnan = false, as the reference) and
- * gathers `obj_bytes`-sized objects into that order, in place in `objs`.
- * `keys` is overwritten with the sorted keys.
+ * argsorts the keys ascending (hasnan = false, as the reference) and gathers
+ * the `obj_bytes`-sized objects into that order, in place in `objs`.  `keys`
+ * is left unchanged.
  */
-XSS_API int xss_gpu_object_sort_by_keys_f64(void *objs, size_t obj_bytes, double *keys, size_t n,
-                                    xss_stream_t stream);
-XSS_API int xss_gpu_object_sort_by_keys_f32(void *objs, size_t obj_bytes, float *keys, size_t n,
-                                    xss_stream_t stream);
-XSS_API int xss_gpu_object_sort_by_keys_u64(void *objs, size_t obj_bytes, uint64_t *keys, size_t n,
-                                    xss_stream_t stream);
-XSS_API int xss_gpu_object_sort_by_keys_i64(void *objs, size_t obj_bytes, int64_t *keys, size_t n,
-                                    xss_stream_t stream);
-XSS_API int xss_gpu_object_sort_by_keys_u32(void *objs, size_t obj_bytes, uint32_t *keys, size_t n,
-                                    xss_stream_t stream);
-XSS_API int xss_gpu_object_sort_by_keys_i32(void *objs, size_t obj_bytes, int32_t *keys, size_t n,
-                                    xss_stream_t stream);
+XSS_API int xss_gpu_object_sort_by_keys_f64(void *objs, size_t obj_bytes, const double *keys, size_t n,
+                                            xss_stream_t stream);
+XSS_API int xss_gpu_object_sort_by_keys_f32(void *objs, size_t obj_bytes, const float *keys, size_t n,
+                                            xss_stream_t stream);
+XSS_API int xss_gpu_object_sort_by_keys_u64(void *objs, size_t obj_bytes, const uint64_t *keys, size_t n,
+                                            xss_stream_t stream);
+XSS_API int xss_gpu_object_sort_by_keys_i64(void *objs, size_t obj_bytes, const int64_t *keys, size_t n,
+                                            xss_stream_t stream);
+XSS_API int xss_gpu_object_sort_by_keys_u32(void *objs, size_t obj_bytes, const uint32_t *keys, size_t n,
+                                            xss_stream_t stream);
+XSS_API int xss_gpu_object_sort_by_keys_i32(void *objs, size_t obj_bytes, const int32_t *keys, size_t n,
+                                            xss_stream_t stream);
+/* object_qsort over {double x, y, z} keyed by sqrt(x*x + y*y + z*z), with
+ * the key computed on the device in IEEE double arithmetic without
+ * contraction (the same value a host computes with -ffp-contract=off);
+ * keys_out (n doubles, may be NULL) receives the keys in sorted order.
+ * The Euclidean-distance workload of the reference's object_qsort
+ * benchmark (benchmarks/bench-objsort.hpp). */
+XSS_API int xss_gpu_object_qsort_xyz_norm(double *xyz, size_t n, double *keys_out, xss_stream_t stream);
 
 /*
  * Host-pointer C symbols of the reference (lib/x86simdsort.cpp:298-339):

@@ -1,0 +1,69 @@
+"""CPU checks of the drop-in boundary: the library loads, every function the
+headers in include/ declare is exported, and the C++ templates resolve to the
+reference's mangled names (no GPU calls here)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import xss_testlib as T
+
+LIB = os.path.join(T.REPO, "x86-simd-sort_amd", "libxss_amd.so")
+INC = os.path.join(T.REPO, "include")
+
+
+def exported():
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True, check=True).stdout
+    return {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+
+
+def test_library_loads():
+    L = ctypes.CDLL(LIB)
+    L.xss_gpu_version.restype = ctypes.c_char_p
+    assert L.xss_gpu_version().startswith(b"xss-amd")
+
+
+def test_every_c_declaration_is_exported():
+    pre = subprocess.run(["gcc", "-E", "-P", "-x", "c", os.path.join(INC, "xss_gpu.h")], capture_output=True,
+                         text=True, check=True).stdout
+    names = set(re.findall(r"\b((?:xss_gpu_|keyvalue_qsort_)\w+)\s*\(", pre))
+    assert len(names) >= 174, len(names)
+    missing = sorted(names - exported())
+    assert not missing, missing
+
+
+def test_cpp_templates_resolve_to_library_symbols(tmp_path):
+    """Instantiate every template the reference exports (lib/x86simdsort.cpp)
+    from a translation unit compiled by g++ against include/x86simdsort.h and
+    check that each symbol it needs is defined by the library."""
+    types = ["uint16_t", "int16_t", "float", "double", "uint32_t", "int32_t", "uint64_t", "int64_t"]
+    kv = ["uint32_t", "int32_t", "float", "uint64_t", "int64_t", "double"]
+    src = ['#include "x86simdsort.h"', "void use() {"]
+    for t in types:
+        src += [f"{{ {t} *a = nullptr; x86simdsort::qsort(a, 0); x86simdsort::qselect(a, 0, 0);",
+                f"  x86simdsort::partial_qsort(a, 0, 0); x86simdsort::argsort(a, 0); x86simdsort::argselect(a, 0, 0); }}"]
+    for k in kv:
+        for v in kv:
+            src += [f"{{ {k} *a = nullptr; {v} *b = nullptr; x86simdsort::keyvalue_qsort(a, b, 0);",
+                    f"  x86simdsort::keyvalue_select(a, b, 0, 0); x86simdsort::keyvalue_partial_sort(a, b, 0, 0); }}"]
+    src += ["}"]
+    f = tmp_path / "use.cpp"
+    f.write_text("\n".join(src))
+    obj = tmp_path / "use.o"
+    subprocess.run(["g++", "-std=c++17", "-c", "-I", INC, str(f), "-o", str(obj)], check=True)
+    und = subprocess.run(["nm", "-u", str(obj)], capture_output=True, text=True, check=True).stdout
+    need = {ln.split()[-1] for ln in und.splitlines() if "x86simdsort" in ln}
+    assert len(need) == 8 * 5 + 36 * 3
+    missing = sorted(need - exported())
+    assert not missing, missing
+
+
+@pytest.mark.skipif(T.ref() is None, reason="oracle/_ref not built")
+def test_exports_cover_the_reference_library():
+    """Every x86simdsort symbol the reference's own build exports."""
+    out = subprocess.run(["nm", "-D", "--defined-only", T.REF_SO], capture_output=True, text=True, check=True).stdout
+    ref_syms = {ln.split()[-1] for ln in out.splitlines() if "x86simdsort" in ln}
+    assert len(ref_syms) >= 150
+    assert not sorted(ref_syms - exported())

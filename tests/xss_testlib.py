@@ -304,14 +304,22 @@ def assert_prefix_equal(got, exp, k, tag, what=""):
         f"{what}: bit patterns of the sorted prefix differ"
 
 
-def assert_kv_equal(gk, gv, ek, ev, tag, what=""):
-    assert_sorted_equal(gk, ek, tag, what + " keys")
+def assert_kv_equal(gk, gv, ek, ev, tag, what="", zeros_as_values=False):
+    """Keys as assert_sorted_equal; (key, value) pairs as a multiset within
+    each run of equal keys (tests/test-keyvalue.cpp:44-100).  With
+    zeros_as_values the key bits inside the +-0 run are compared by value
+    (the reference's min/max networks may rewrite -0.0 as +0.0)."""
+    assert_sorted_equal(gk, ek, tag, what + " keys", zeros_as_values)
     ce = vclass(ek, tag)
     run = np.concatenate([[0], np.cumsum(ce[1:] != ce[:-1])]) if ce.size else ce
+    kg, ke = bits(gk).astype(np.uint64), bits(ek).astype(np.uint64)
+    if zeros_as_values:
+        kg = np.where(vclass(gk, tag) == 0, np.uint64(0), kg)
+        ke = np.where(ce == 0, np.uint64(0), ke)
     # (run, key bits, value bits) multisets must agree
-    og = np.lexsort((bits(gv), bits(gk), run))
-    oe = np.lexsort((bits(ev), bits(ek), run))
-    assert np.array_equal(bits(gk)[og], bits(ek)[oe]), f"{what}: key/value pairing differs"
+    og = np.lexsort((bits(gv), kg, run))
+    oe = np.lexsort((bits(ev), ke, run))
+    assert np.array_equal(kg[og], ke[oe]), f"{what}: key/value pairing differs"
     assert np.array_equal(bits(gv)[og], bits(ev)[oe]), f"{what}: values differ within equal-key runs"
 
 
@@ -322,10 +330,12 @@ def assert_argsort(idx, a, exp_sorted, tag, what=""):
     assert_sorted_equal(a[idx.astype(np.int64)], exp_sorted, tag, what + " gathered keys")
 
 
-def assert_selected(got, inp, exp_sorted, k, tag, what=""):
+def assert_selected(got, inp, exp_sorted, k, tag, what="", zeros_as_values=False):
     """got is a valid select(k) result for input `inp` whose full sort is exp_sorted."""
-    n = inp.size
-    assert np.array_equal(np.sort(bits(got)), np.sort(bits(inp))), f"{what}: not a permutation of the input"
+    if zeros_as_values:
+        assert np.array_equal(np.sort(vclass(got, tag)), np.sort(vclass(inp, tag))), f"{what}: not a permutation"
+    else:
+        assert np.array_equal(np.sort(bits(got)), np.sort(bits(inp))), f"{what}: not a permutation of the input"
     cg, ce = vclass(got, tag), vclass(exp_sorted, tag)
     assert cg[k] == ce[k], f"{what}: k-th element {got[k]} != {exp_sorted[k]}"
     assert np.array_equal(np.sort(cg[:k]), np.sort(ce[:k])), f"{what}: left partition differs"

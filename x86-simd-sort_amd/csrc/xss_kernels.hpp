@@ -117,13 +117,13 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
     const uint32_t v0 = r0 + head;
     const uint32_t nvec = (r1 - v0) / VEC;
     const uint4 *kv = reinterpret_cast<const uint4 *>(keys + v0);
-    for (uint32_t v = threadIdx.x; v < nvec; v += UP_BLOCK) {
-        uint4 q = kv[v];
+    // count one 16-byte vector of keys; whole-wave runs of one digit (sorted
+    // or constant data) add once
+    auto count_vec = [&](const uint4 &q) {
         const U *e = reinterpret_cast<const U *>(&q);
 #pragma unroll
         for (int j = 0; j < VEC; ++j) {
             const uint32_t d = KD::digit(e[j], flip, nl, shift);
-            // whole-wave runs of one digit (sorted or constant data) add once
             const uint64_t act = __ballot(1);
             const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
             if (__ballot(d != d0) == 0) {
@@ -132,7 +132,18 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
                 atomicAdd(&h[d], 1u);
             }
         }
+    };
+    // four loads in flight per thread before any is consumed
+    constexpr uint32_t UNR = 4;
+    uint32_t v = threadIdx.x;
+    for (; v + (UNR - 1) * UP_BLOCK < nvec; v += UNR * UP_BLOCK) {
+        uint4 q[UNR];
+#pragma unroll
+        for (uint32_t u = 0; u < UNR; ++u) q[u] = kv[v + u * UP_BLOCK];
+#pragma unroll
+        for (uint32_t u = 0; u < UNR; ++u) count_vec(q[u]);
     }
+    for (; v < nvec; v += UP_BLOCK) count_vec(kv[v]);
     const uint32_t t0 = v0 + nvec * VEC;
     const uint32_t rest = head + (r1 - t0);
     for (uint32_t r = threadIdx.x; r < rest; r += UP_BLOCK) {
@@ -228,8 +239,25 @@ struct PassArgs {
     const uint32_t *total;   // [256]
 };
 
-template <class KD, int PL, int BLOCK, int ITEMS, int MINW = 1>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW, 8))) void downsweep_kernel(PassArgs a)
+template <class KD, int PL, int BLOCK, int ITEMS>
+struct DownsweepSmem {
+    using U = typename KD::U;
+    using V = typename PayloadT<PL>::type;
+    static constexpr int WAVES = BLOCK / 64;
+    static constexpr int TILE = BLOCK * ITEMS;
+    uint32_t off[WAVES * 256];   // per-wave digit counters, then tile slot bases
+    uint32_t base[256];          // global position of tile slot 0 of each digit
+    uint32_t wsum[WAVES < 4 ? 4 : WAVES];
+    U keys[TILE];                // the tile, sorted by digit
+    uint8_t dig[TILE];           // digit of each sorted slot
+    V vals[PL == PL_NONE ? 1 : TILE];
+};
+
+// One tile of a downsweep: load, rank, local sort, write-out.  FULL: all
+// TILE keys valid (no per-item bounds checks); NL: floats with hasnan.
+template <class KD, int PL, int BLOCK, int ITEMS, bool FULL, bool NL>
+__device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<KD, PL, BLOCK, ITEMS> &sm,
+                                               uint32_t tile, uint32_t &run)
 {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
@@ -237,142 +265,133 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(MINW, 8))
     constexpr int WAVES = BLOCK / 64;
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int WTILE = 64 * ITEMS;
-    static_assert(BLOCK >= 256 && BLOCK % 64 == 0, "one thread per digit needed");
-    static_assert(WTILE <= 65535, "rank packing");
-
-    __shared__ uint32_t s_cnt[WAVES * 256];
-    __shared__ uint32_t s_start[256];
-    __shared__ uint32_t s_base[256];
-    __shared__ uint32_t s_wsum[WAVES < 4 ? 4 : WAVES];
-    __shared__ U s_keys[TILE];
-    __shared__ V s_vals[HAS_PL ? TILE : 1];
-
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const U flip = U(a.flip);
-    const bool nl = a.nan_last != 0;
     const int shift = int(a.shift);
     const U *kin = static_cast<const U *>(a.keys_in);
     const V *vin = static_cast<const V *>(a.vals_in);
+
+    for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) sm.off[i] = 0;
+    const uint32_t tbase = tile * TILE;
+    const uint32_t valid = FULL ? uint32_t(TILE) : min(uint32_t(TILE), a.n - tbase);
+    U key[ITEMS];
+    V val[ITEMS];
+    const uint32_t lbase = wave * WTILE + lane;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t lp = lbase + i * 64;
+        if (FULL || lp < valid) {
+            key[i] = kin[tbase + lp];
+            if constexpr (PL == PL_IDX) val[i] = a.first ? (tbase + lp) : vin[tbase + lp];
+            else if constexpr (HAS_PL) val[i] = vin[tbase + lp];
+        } else {
+            key[i] = U(0);
+            if constexpr (HAS_PL) val[i] = V(0);
+        }
+    }
+    __syncthreads();  // counters cleared; the previous tile's LDS reads are done
+
+    // --- rank: stable within the tile (wave-striped order), per-wave counters
+    uint32_t rd[ITEMS];  // rank in the wave's digit run (low 16 bits), digit (bits 16..23)
+    uint32_t *wc = sm.off + wave * 256;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t lp = lbase + i * 64;
+        const uint32_t d = (FULL || lp < valid) ? KD::digit(key[i], flip, NL, shift) : 255u;
+        const uint64_t peers = match_any8(d);
+        const uint32_t below = lanes_below(peers);
+        const uint32_t tot = uint32_t(__popcll(peers));
+        const uint32_t c = wc[d];
+        rd[i] = (c + below) | (d << 16);
+        if (below + 1 == tot) wc[d] = c + tot;
+    }
+    __syncthreads();
+
+    // --- digit counts of the tile, their exclusive scan, and the slot base of
+    // every (wave, digit): off[w][d] = start of digit d in the tile + the
+    // counts of d in waves < w
+    uint32_t tcount = 0;
+    if (tid < 256) {
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) tcount += sm.off[w * 256 + tid];
+    }
+    uint32_t incl = tcount;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off);
+        if (lane >= uint32_t(off)) incl += t;
+    }
+    if (lane == 63) sm.wsum[wave] = incl;
+    __syncthreads();
+    if (tid < 256) {
+        uint32_t st = incl - tcount;
+        for (uint32_t w = 0; w < wave; ++w) st += sm.wsum[w];
+        sm.base[tid] = run - st;
+        // padding of a partial last tile was ranked as digit 255 (last)
+        run += tcount - ((!FULL && tid == 255) ? (TILE - valid) : 0u);
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            const uint32_t c = sm.off[w * 256 + tid];
+            sm.off[w * 256 + tid] = st;
+            st += c;
+        }
+    }
+    __syncthreads();
+
+    // --- tile-local sort: scatter into LDS at slot base + rank
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t d = rd[i] >> 16;
+        const uint32_t lpos = wc[d] + (rd[i] & 0xffffu);
+        sm.keys[lpos] = key[i];
+        sm.dig[lpos] = uint8_t(d);
+        if constexpr (HAS_PL) sm.vals[lpos] = val[i];
+    }
+    __syncthreads();
+
+    // --- write-out: consecutive threads take consecutive sorted slots
     U *kout = static_cast<U *>(a.keys_out);
     V *vout = static_cast<V *>(a.vals_out);
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t lpos = i * BLOCK + tid;
+        if (FULL || lpos < valid) {
+            const uint32_t g = sm.base[sm.dig[lpos]] + lpos;
+            if constexpr (PL == PL_IDX) {
+                if (a.last) {
+                    a.idx_out[g] = uint64_t(sm.vals[lpos]);
+                    continue;
+                }
+            }
+            const U x = sm.keys[lpos];
+            if constexpr (KD::KIND == Kind::Float && NL) {
+                kout[g] = (a.canon_nan && KD::is_nan(x)) ? KD::QNAN : x;
+            } else {
+                kout[g] = x;
+            }
+            if constexpr (HAS_PL) vout[g] = sm.vals[lpos];
+        }
+    }
+}
+
+template <class KD, int PL, int BLOCK, int ITEMS, bool NL>
+__global__ __launch_bounds__(BLOCK) void downsweep_kernel(PassArgs a)
+{
+    constexpr int TILE = BLOCK * ITEMS;
+    static_assert(BLOCK >= 256 && BLOCK % 64 == 0, "one thread per digit needed");
+    static_assert(64 * ITEMS <= 65535, "rank packing");
+    __shared__ DownsweepSmem<KD, PL, BLOCK, ITEMS> sm;
+    const uint32_t tid = threadIdx.x;
     const uint32_t ntiles = (a.n + TILE - 1) / TILE;
     const uint32_t tile0 = blockIdx.x * a.range_tiles;
     const uint32_t tile1 = min(ntiles, tile0 + a.range_tiles);
-    // running global base of each digit of this range (thread d owns digit d)
-    const uint32_t dstart = digit_starts(a.total, s_wsum);
+    // running global position of each digit of this range (thread d owns digit d)
+    const uint32_t dstart = digit_starts(a.total, sm.wsum);
     uint32_t run = tid < 256 ? dstart + a.rpre[size_t(tid) * a.ranges + blockIdx.x] : 0u;
-
-    for (uint32_t tile = tile0; tile < tile1; ++tile) {
-        for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) s_cnt[i] = 0;
-        const uint32_t tbase = tile * TILE;
-        const uint32_t valid = min(uint32_t(TILE), a.n - tbase);
-        U key[ITEMS];
-        V val[ITEMS];
-        const uint32_t lbase = wave * WTILE + lane;
-        if (valid == TILE) {
-#pragma unroll
-            for (int i = 0; i < ITEMS; ++i) {
-                const uint32_t lp = lbase + i * 64;
-                key[i] = kin[tbase + lp];
-                if constexpr (PL == PL_IDX) val[i] = a.first ? (tbase + lp) : vin[tbase + lp];
-                else if constexpr (HAS_PL) val[i] = vin[tbase + lp];
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < ITEMS; ++i) {
-                const uint32_t lp = lbase + i * 64;
-                key[i] = U(0);
-                if constexpr (HAS_PL) val[i] = V(0);
-                if (lp < valid) {
-                    key[i] = kin[tbase + lp];
-                    if constexpr (PL == PL_IDX) val[i] = a.first ? (tbase + lp) : vin[tbase + lp];
-                    else if constexpr (HAS_PL) val[i] = vin[tbase + lp];
-                }
-            }
-        }
-        __syncthreads();  // s_cnt cleared; previous tile's LDS reads done
-
-        // --- rank: stable within the tile (wave-striped order), per-wave counters
-        uint32_t rd[ITEMS];  // rank in low 16 bits, digit in bits 16..23
-        uint32_t *wc = s_cnt + wave * 256;
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-            const uint32_t lp = lbase + i * 64;
-            const uint32_t d = (lp < valid) ? KD::digit(key[i], flip, nl, shift) : 255u;
-            const uint64_t peers = match_any8(d);
-            const uint32_t below = lanes_below(peers);
-            const uint32_t tot = uint32_t(__popcll(peers));
-            const uint32_t c = wc[d];
-            rd[i] = (c + below) | (d << 16);
-            if (below + 1 == tot) wc[d] = c + tot;
-        }
-        __syncthreads();
-
-        // --- per-digit tile counts and wave prefixes
-        uint32_t tcount = 0;
-        if (tid < 256) {
-            uint32_t s = 0;
-#pragma unroll
-            for (int w = 0; w < WAVES; ++w) {
-                const uint32_t c = s_cnt[w * 256 + tid];
-                s_cnt[w * 256 + tid] = s;
-                s += c;
-            }
-            tcount = s;
-        }
-        // exclusive scan of tcount over the 256 digits (tile-local bucket starts)
-        uint32_t incl = tcount;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t t = __shfl_up(incl, off);
-            if (lane >= uint32_t(off)) incl += t;
-        }
-        if (lane == 63) s_wsum[wave] = incl;
-        __syncthreads();
-        if (tid < 256) {
-            uint32_t pre = 0;
-            for (uint32_t w = 0; w < wave; ++w) pre += s_wsum[w];
-            const uint32_t st = pre + incl - tcount;
-            s_start[tid] = st;
-            s_base[tid] = run - st;
-            // padding of a partial last tile was ranked as digit 255 (last)
-            run += tcount - (tid == 255 ? (TILE - valid) : 0u);
-        }
-        __syncthreads();
-
-        // --- tile-local sort: scatter into LDS at bucket start + wave prefix + rank
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-            const uint32_t d = rd[i] >> 16;
-            const uint32_t lpos = s_start[d] + wc[d] + (rd[i] & 0xffffu);
-            s_keys[lpos] = key[i];
-            if constexpr (HAS_PL) s_vals[lpos] = val[i];
-        }
-        __syncthreads();
-
-        // --- write-out: consecutive threads take consecutive sorted positions
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-            const uint32_t lpos = i * BLOCK + tid;
-            if (lpos < valid) {
-                const U x = s_keys[lpos];
-                const uint32_t d = KD::digit(x, flip, nl, shift);
-                const uint32_t g = s_base[d] + lpos;
-                if constexpr (PL == PL_IDX) {
-                    if (a.last) {
-                        a.idx_out[g] = uint64_t(s_vals[lpos]);
-                        continue;
-                    }
-                }
-                if constexpr (KD::KIND == Kind::Float) {
-                    kout[g] = (a.canon_nan && KD::is_nan(x)) ? KD::QNAN : x;
-                } else {
-                    kout[g] = x;
-                }
-                if constexpr (HAS_PL) vout[g] = s_vals[lpos];
-            }
-        }
-    }
+    const uint32_t nfull = min(tile1, a.n / TILE);
+    uint32_t tile = tile0;
+    for (; tile < nfull; ++tile) downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(a, sm, tile, run);
+    for (; tile < tile1; ++tile) downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, tile, run);
 }
 
 // ---------------------------------------------------------------- block sort

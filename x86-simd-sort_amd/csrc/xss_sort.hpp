@@ -41,7 +41,7 @@ template <class KD, int PL>
 struct PassShape {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
-    static constexpr int BLOCK = 256;
+    static constexpr int BLOCK = 512;
     static constexpr int ITEMS = (sizeof(U) + (PL == PL_NONE ? 0 : sizeof(V))) > 8 ? 8 : 16;
     static constexpr int TILE = BLOCK * ITEMS;
 };
@@ -161,8 +161,12 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
         }
         {
             KernelTimer t("downsweep", s);
-            hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS>), dim3(ranges), dim3(PS::BLOCK), 0,
-                               s, a);
+            if (nan_last)
+                hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, true>), dim3(ranges),
+                                   dim3(PS::BLOCK), 0, s, a);
+            else
+                hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, false>), dim3(ranges),
+                                   dim3(PS::BLOCK), 0, s, a);
         }
         XSS_HIP_CHECK(hipGetLastError());
         src_k = a.keys_out;

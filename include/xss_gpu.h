@@ -143,6 +143,38 @@ XSS_API int xss_gpu_object_sort_by_keys_i32(void *objs, size_t obj_bytes, const 
 XSS_API int xss_gpu_object_qsort_xyz_norm(double *xyz, size_t n, double *keys_out, xss_stream_t stream);
 
 /*
+ * Segmented form (no counterpart in the reference, which sorts one array per
+ * call): many independent arrays, stored back to back in one device buffer,
+ * sorted / argsorted / selected by one pipeline.  `offsets` (host memory,
+ * nseg + 1 entries, offsets[0] = 0) delimit the segments.  A plan buckets the
+ * segments by size once and owns all the device memory its runs need, so
+ * xss_gpu_segplan_run allocates nothing and may be captured in a hipGraph.
+ *   op        "qsort"   in-place sort of every segment (qsort semantics);
+ *             "argsort" writes segment-local uint32 indices to idx_out;
+ *             "qselect" every segment left partitioned for any k (it is
+ *                       sorted, with NaN bits kept, which satisfies
+ *                       qselect's contract for every k at once)
+ *   key_type  "u16" "i16" "f16" "u32" "i32" "f32" "u64" "i64" "f64"
+ *   small_max segments up to this length use one LDS workgroup each
+ *             (0 = the largest the type allows); range_tiles: downsweep
+ *             tiles per workgroup in larger segments (0 = 8).
+ */
+typedef struct xss_segplan *xss_segplan_t;
+XSS_API int xss_gpu_segplan_create(xss_segplan_t *plan, const char *op, const char *key_type,
+                                   const uint64_t *offsets, size_t nseg, uint32_t small_max, uint32_t range_tiles);
+XSS_API int xss_gpu_segplan_run(xss_segplan_t plan, void *keys, uint32_t *idx_out, int hasnan, int descending,
+                                xss_stream_t stream);
+XSS_API int xss_gpu_segplan_info(xss_segplan_t plan, uint32_t *nsmall, uint32_t *nranges, uint32_t *nlarge);
+XSS_API int xss_gpu_segplan_destroy(xss_segplan_t plan);
+/* Baselines: one plain call per segment, enqueued in a C++ loop. */
+XSS_API int xss_gpu_loop_qsort_f32(float *keys, const uint64_t *offsets, size_t nseg, int hasnan, int descending,
+                                   xss_stream_t stream);
+XSS_API int xss_gpu_loop_argsort_i32(const int32_t *keys, uint64_t *idx, const uint64_t *offsets, size_t nseg,
+                                     xss_stream_t stream);
+XSS_API int xss_gpu_loop_qselect_f32(float *keys, const uint64_t *offsets, size_t nseg, int hasnan,
+                                     xss_stream_t stream);
+
+/*
  * Host-pointer C symbols of the reference (lib/x86simdsort.cpp:298-339):
  * ascending key-value sort with hasnan = true, synchronous.
  */

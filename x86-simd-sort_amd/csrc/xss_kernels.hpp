@@ -95,6 +95,8 @@ struct UpArgs {
     uint64_t flip;
     uint32_t *rsum;        // [256][ranges] (digit-major)
     uint32_t ranges;
+    const uint64_t *range_begin;  // segmented form: explicit key range of
+    const uint64_t *range_end;    // each workgroup (null: blockIdx.x * range_keys)
 };
 
 template <class KD>
@@ -108,8 +110,8 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
     const U flip = U(a.flip);
     const bool nl = a.nan_last != 0;
     const int shift = int(a.shift);
-    const uint32_t r0 = blockIdx.x * a.range_keys;
-    const uint32_t r1 = min(a.n, r0 + a.range_keys);
+    const uint32_t r0 = a.range_begin ? uint32_t(a.range_begin[blockIdx.x]) : blockIdx.x * a.range_keys;
+    const uint32_t r1 = a.range_begin ? uint32_t(a.range_end[blockIdx.x]) : min(a.n, r0 + a.range_keys);
     constexpr int VEC = 16 / sizeof(U);
     // keys before the first 16-byte boundary are read one by one
     const uint32_t mis = uint32_t(reinterpret_cast<uintptr_t>(keys + r0) & 15u);
@@ -225,7 +227,8 @@ struct PassArgs {
     void *keys_out;
     const void *vals_in;
     void *vals_out;
-    uint64_t *idx_out;       // PL_IDX, last pass
+    uint64_t *idx_out;       // PL_IDX, last pass (size_t indices)
+    uint32_t *idx32_out;     // PL_IDX, last pass: segment-local 32-bit indices instead
     uint32_t n;
     uint32_t shift;
     uint64_t flip;
@@ -246,7 +249,7 @@ struct DownsweepSmem {
     static constexpr int WAVES = BLOCK / 64;
     static constexpr int TILE = BLOCK * ITEMS;
     uint32_t off[WAVES * 256];   // per-wave digit counters, then tile slot bases
-    uint32_t base[256];          // global position of tile slot 0 of each digit
+    uint64_t base[256];          // global position of tile slot 0 of each digit
     uint32_t wsum[WAVES < 4 ? 4 : WAVES];
     U keys[TILE];                // the tile, sorted by digit
     uint8_t dig[TILE];           // digit of each sorted slot
@@ -255,9 +258,12 @@ struct DownsweepSmem {
 
 // One tile of a downsweep: load, rank, local sort, write-out.  FULL: all
 // TILE keys valid (no per-item bounds checks); NL: floats with hasnan.
+// The tile holds keys [tbase, tbase + valid) (valid == TILE when FULL).
+// seg_base: start of the segment the tile belongs to (segment-local argsort
+// indices are position - seg_base).
 template <class KD, int PL, int BLOCK, int ITEMS, bool FULL, bool NL>
 __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<KD, PL, BLOCK, ITEMS> &sm,
-                                               uint32_t tile, uint32_t &run)
+                                               uint64_t tbase, uint32_t valid_in, uint64_t seg_base, uint64_t &run)
 {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
@@ -272,8 +278,7 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
     const V *vin = static_cast<const V *>(a.vals_in);
 
     for (uint32_t i = tid; i < WAVES * 256; i += BLOCK) sm.off[i] = 0;
-    const uint32_t tbase = tile * TILE;
-    const uint32_t valid = FULL ? uint32_t(TILE) : min(uint32_t(TILE), a.n - tbase);
+    const uint32_t valid = FULL ? uint32_t(TILE) : valid_in;
     U key[ITEMS];
     V val[ITEMS];
     const uint32_t lbase = wave * WTILE + lane;
@@ -282,7 +287,7 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
         const uint32_t lp = lbase + i * 64;
         if (FULL || lp < valid) {
             key[i] = kin[tbase + lp];
-            if constexpr (PL == PL_IDX) val[i] = a.first ? (tbase + lp) : vin[tbase + lp];
+            if constexpr (PL == PL_IDX) val[i] = a.first ? uint32_t(tbase + lp) : vin[tbase + lp];
             else if constexpr (HAS_PL) val[i] = vin[tbase + lp];
         } else {
             key[i] = U(0);
@@ -356,10 +361,11 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
     for (int i = 0; i < ITEMS; ++i) {
         const uint32_t lpos = i * BLOCK + tid;
         if (FULL || lpos < valid) {
-            const uint32_t g = sm.base[sm.dig[lpos]] + lpos;
+            const uint64_t g = sm.base[sm.dig[lpos]] + lpos;
             if constexpr (PL == PL_IDX) {
                 if (a.last) {
-                    a.idx_out[g] = uint64_t(sm.vals[lpos]);
+                    if (a.idx32_out) a.idx32_out[g] = uint32_t(sm.vals[lpos] - uint32_t(seg_base));
+                    else a.idx_out[g] = uint64_t(sm.vals[lpos]);
                     continue;
                 }
             }
@@ -387,11 +393,13 @@ __global__ __launch_bounds__(BLOCK) void downsweep_kernel(PassArgs a)
     const uint32_t tile1 = min(ntiles, tile0 + a.range_tiles);
     // running global position of each digit of this range (thread d owns digit d)
     const uint32_t dstart = digit_starts(a.total, sm.wsum);
-    uint32_t run = tid < 256 ? dstart + a.rpre[size_t(tid) * a.ranges + blockIdx.x] : 0u;
+    uint64_t run = tid < 256 ? dstart + a.rpre[size_t(tid) * a.ranges + blockIdx.x] : 0u;
     const uint32_t nfull = min(tile1, a.n / TILE);
     uint32_t tile = tile0;
-    for (; tile < nfull; ++tile) downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(a, sm, tile, run);
-    for (; tile < tile1; ++tile) downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, tile, run);
+    for (; tile < nfull; ++tile)
+        downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(a, sm, uint64_t(tile) * TILE, TILE, 0, run);
+    for (; tile < tile1; ++tile)
+        downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, uint64_t(tile) * TILE, a.n - tile * TILE, 0, run);
 }
 
 // ---------------------------------------------------------------- block sort
@@ -402,6 +410,7 @@ struct BlockSortArgs {
     const void *vals_in;
     void *vals_out;
     uint64_t *idx_out;         // PL_IDX
+    uint32_t *idx32_out;       // PL_IDX: segment-local 32-bit indices instead
     const uint64_t *seg_off;   // segment s = [seg_off[s], seg_off[s+1]); null: one array [0, n)
     const uint32_t *seg_list;  // segment ids handled by this launch (null: blockIdx.x)
     uint32_t n;
@@ -433,7 +442,8 @@ __global__ __launch_bounds__(BLOCK) void block_sort_kernel(BlockSortArgs a)
             const U *kin = static_cast<const U *>(a.keys_in);
             if (threadIdx.x == 0) {
                 if constexpr (PL == PL_IDX) {
-                    a.idx_out[base] = a.idx_local ? 0 : base;
+                    if (a.idx32_out) a.idx32_out[base] = 0;
+                    else a.idx_out[base] = a.idx_local ? 0 : base;
                 } else {
                     U x = kin[base];
                     if constexpr (KD::KIND == Kind::Float)
@@ -484,7 +494,8 @@ __global__ __launch_bounds__(BLOCK) void block_sort_kernel(BlockSortArgs a)
     for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
         const uint32_t src = s_idx[i];
         if constexpr (PL == PL_IDX) {
-            a.idx_out[base + i] = a.idx_local ? uint64_t(src) : base + src;
+            if (a.idx32_out) a.idx32_out[base + i] = src;
+            else a.idx_out[base + i] = a.idx_local ? uint64_t(src) : base + src;
         } else {
             U x = s_raw[src];
             if constexpr (KD::KIND == Kind::Float)

@@ -24,6 +24,9 @@ The JSON line carries:
                       (1 + 2P) * key bytes per key of an 8-bit LSD sort; this
                       design moves 3P * key bytes per key (each pass = one
                       upsweep read + one downsweep read and write)
+  segmented           the batch form: 8192 uneven float32 arrays (~775M keys)
+                      sorted by one graph-captured plan against a loop of
+                      plain per-array calls on the same GPU
   cpu_baseline        the reference library (oracle/_ref, AVX-512 dispatch +
                       OpenMP, <= 16 threads) timed on this host on a bounded
                       sample of the same distribution
@@ -92,6 +95,107 @@ def cpu_baseline(dtype, seconds=12.0):
                       f"(input restore excluded); dispatch {isa}"}
 
 
+def segmented_batch(torch, xss_amd, L, tuning, reps=3):
+    """8192 float32 arrays, lengths log-uniform in [16, 2^20] (seeded), values
+    U(-1, 1) with 0.1% NaN, hasnan = true: one plan captured in a graph and
+    replayed, against a C++ loop of plain xss_gpu_qsort_f32 calls on the same
+    GPU (inputs restored outside the timed region).  Also per-segment int32
+    argsort (uint32 indices) and select at k = len/2."""
+    import numpy as np
+    rng = np.random.default_rng(2024)
+    nseg = 8192
+    sizes = np.exp(rng.uniform(np.log(16), np.log(1 << 20), nseg)).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    n = int(off[-1])
+    g = torch.Generator(device="cuda")
+    g.manual_seed(77)
+    vals = torch.rand(n, device="cuda", generator=g) * 2 - 1
+    vals[torch.rand(n, device="cuda", generator=g) < 0.001] = float("nan")
+    pristine = vals.view(torch.int32).clone()
+    del vals
+    ints = torch.randint(-2 ** 31, 2 ** 31, (n,), dtype=torch.int32, device="cuda", generator=g)
+    work = torch.empty_like(pristine)
+    stream = torch.cuda.current_stream()
+    offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+    def cur():  # the stream torch launches on now (a capture stream inside torch.cuda.graph)
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def plan(op, tag):
+        p = ctypes.c_void_p()
+        rc = L.xss_gpu_segplan_create(ctypes.byref(p), op.encode(), tag.encode(), offp, ctypes.c_size_t(nseg),
+                                      ctypes.c_uint32(tuning.get("small_max", 0)),
+                                      ctypes.c_uint32(tuning.get("range_tiles", 0)))
+        assert rc == 0, L.xss_gpu_last_error()
+        return p
+
+    def timed(fn, src=None):
+        ts = []
+        for _ in range(reps):
+            work.copy_(pristine if src is None else src)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return min(ts)
+
+    def check_sorted():
+        v = work.view(torch.float32)
+        bad = 0
+        for i in range(0, nseg, 512):  # spot-check every 512th segment
+            a, b = int(off[i]), int(off[i + 1])
+            seg = v[a:b]
+            fin = seg[~torch.isnan(seg)]
+            bad += int((fin[1:] < fin[:-1]).sum().item()) + int(torch.isnan(seg[: fin.numel()]).sum().item())
+        return bad == 0
+
+    out = {"workload": "8192 f32 arrays, len log-uniform [16, 2^20], U(-1,1) with 0.1% NaN, hasnan=true",
+           "n_keys": n, "n_segments": nseg, "tuning": tuning}
+    p = plan("qsort", "f32")
+    ptr = ctypes.c_void_p(work.data_ptr())
+    run = lambda: L.xss_gpu_segplan_run(p, ptr, None, 1, 0, cur())  # noqa: E731
+    run()
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        run()
+    t_graph = timed(graph.replay)
+    ok = check_sorted()
+    t_loop = timed(lambda: L.xss_gpu_loop_qsort_f32(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_float)), offp,
+                                                    ctypes.c_size_t(nseg), 1, 0, cur()))
+    out.update({"segmented_ms": round(t_graph, 3), "loop_ms": round(t_loop, 3),
+                "speedup_vs_loop": round(t_loop / t_graph, 2), "segmented_Gkeys_s": round(n / t_graph / 1e6, 2),
+                "sorted_ok": ok})
+    L.xss_gpu_segplan_destroy(p)
+    # argsort of int32 keys with uint32 indices
+    idx = torch.empty(n, dtype=torch.int32, device="cuda")
+    p = plan("argsort", "i32")
+    ia = lambda: L.xss_gpu_segplan_run(p, ptr, ctypes.c_void_p(idx.data_ptr()), 0, 0, cur())  # noqa: E731
+    work.copy_(ints)
+    ia()
+    t_arg = timed(ia, ints)
+    idx64 = torch.empty(n, dtype=torch.int64, device="cuda")
+    t_arg_loop = timed(lambda: L.xss_gpu_loop_argsort_i32(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_int32)),
+                                                          ctypes.cast(ctypes.c_void_p(idx64.data_ptr()),
+                                                                      ctypes.POINTER(ctypes.c_uint64)),
+                                                          offp, ctypes.c_size_t(nseg), cur()), ints)
+    L.xss_gpu_segplan_destroy(p)
+    p = plan("qselect", "f32")
+    sel = lambda: L.xss_gpu_segplan_run(p, ptr, None, 1, 0, cur())  # noqa: E731
+    sel()
+    t_sel = timed(sel)
+    t_sel_loop = timed(lambda: L.xss_gpu_loop_qselect_f32(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_float)), offp,
+                                                          ctypes.c_size_t(nseg), 1, cur()))
+    L.xss_gpu_segplan_destroy(p)
+    out.update({"argsort_i32_ms": round(t_arg, 3), "argsort_i32_loop_ms": round(t_arg_loop, 3),
+                "qselect_ms": round(t_sel, 3), "qselect_loop_ms": round(t_sel_loop, 3)})
+    del work, pristine, ints, idx, idx64
+    torch.cuda.empty_cache()
+    return out
+
+
 def traffic_from_profiles(dtype, n):
     p = os.path.join(REPO, "profiles", f"downsweep_{dtype}_256M_traffic.json")
     if not os.path.exists(p):
@@ -113,6 +217,7 @@ def main():
     ap.add_argument("--dtype", default="f32", choices=["f32", "u32"])
     ap.add_argument("--n", type=int, default=1 << 28)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-seg", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X, and the sort of one array "
@@ -204,6 +309,13 @@ def main():
         "wall_s_timed_loop_incl_restore": round(wall, 4),
         "sorted_ok": ok,
     }
+    if not args.no_seg:
+        tuning = {}
+        tp = os.path.join(REPO, "x86-simd-sort_amd", "tuning.json")
+        if os.path.exists(tp):
+            tuning = json.load(open(tp)).get("segmented_f32", {})
+        line["segmented"] = segmented_batch(torch, xss_amd, L, tuning)
+        line["segmented_speedup_vs_loop"] = line["segmented"]["speedup_vs_loop"]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(dt)
     print(json.dumps(line))

@@ -175,6 +175,45 @@ XSS_API int xss_gpu_loop_qselect_f32(float *keys, const uint64_t *offsets, size_
                                      xss_stream_t stream);
 
 /*
+ * Serving form (no counterpart in the reference): a continuous-batching
+ * request loop.  A worker thread owns `nstreams` HIP streams, admits queued
+ * requests in arrival order while a stream is free and the admitted HBM
+ * footprint (keys + values + scratch) stays within budget_bytes, coalesces
+ * queued qsort requests of at most coalesce_max keys (and short enough for
+ * one LDS workgroup) with the same key type and flags into one launch, and
+ * runs every other request through its entry point above on its own stream,
+ * so requests overlap on the device.  reserve_bytes grows each stream's
+ * pooled workspace up front.  one_at_a_time = 1 serves requests strictly
+ * one by one (the plain-call baseline).  Requests work in place on the
+ * caller's device buffers, which must stay valid until the request is done.
+ */
+typedef struct xss_request {
+    int op;            /* 0 qsort, 1 keyvalue_qsort, 2 qselect, 3 partial_qsort */
+    int key_type;      /* 0 u16, 1 i16, 2 f16, 3 u32, 4 i32, 5 f32, 6 u64, 7 i64, 8 f64 */
+    int val_type;      /* keyvalue_qsort: value type, same codes */
+    int hasnan;
+    int descending;
+    int reserved;
+    void *keys;        /* device pointers */
+    void *vals;
+    uint64_t n;
+    uint64_t k;        /* qselect / partial_qsort */
+    double arrival_us; /* xss_server_run_trace: submit time after the call starts */
+    double latency_us; /* out: submission to observed completion, -1 on failure */
+    void *internal;
+} xss_request;
+typedef struct xss_server *xss_server_t;
+XSS_API int xss_server_create(xss_server_t *server, size_t budget_bytes, int nstreams, uint32_t coalesce_max,
+                              size_t reserve_bytes, int one_at_a_time);
+XSS_API int xss_server_submit(xss_server_t server, xss_request *req);
+XSS_API int xss_server_wait(xss_server_t server, xss_request *req);
+/* Submits reqs[i] at arrival_us after the call starts (open loop), then
+ * waits for all of them. */
+XSS_API int xss_server_run_trace(xss_server_t server, xss_request *reqs, size_t count);
+XSS_API int xss_server_stats(xss_server_t server, uint64_t *completed, uint64_t *launch_groups, uint64_t *coalesced);
+XSS_API int xss_server_destroy(xss_server_t server);
+
+/*
  * Host-pointer C symbols of the reference (lib/x86simdsort.cpp:298-339):
  * ascending key-value sort with hasnan = true, synchronous.
  */

@@ -1,0 +1,170 @@
+"""Serving form: every request's result in a concurrent, coalesced mix equals
+its result when run alone through the plain call.
+
+A seeded mix of qsort (all key types, short and long, NaN / descending
+flags), keyvalue_qsort, qselect and partial_qsort requests is served three
+ways -- a burst into 4 streams with coalescing, a 2-stream server whose
+admission budget holds only a few requests at a time, and one request at a
+time -- and compared with the same requests run alone:
+  * qsort / keyvalue_qsort: bit-exact (both paths are stable, and qsort(hasnan)
+    writes the canonical NaN);
+  * qselect: the same value class at k, the same multisets left and right of
+    k, and a permutation of the input (the order inside a partition is not
+    fixed);
+  * partial_qsort: the sorted prefix as the partial-sort checker compares it,
+    and the same multiset overall.
+The alone results themselves are checked against the oracle by the sort and
+select test files.
+"""
+import numpy as np
+import pytest
+
+import xss_testlib as T
+
+pytestmark = pytest.mark.gpu
+
+SMALL_TAGS = ["f32", "u64", "i16"]
+KV_VAL = ["u32", "u64", "f64", "i32"]
+DISTS = ["random", "dups", "signed_zeros", "nan", "nan_payloads", "sorted"]
+
+
+def _mix(count, seed):
+    """[(op, tag, vtag, keys, vals, k, hasnan, desc)]"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        u = rng.random()
+        dist = DISTS[int(rng.integers(0, len(DISTS)))]
+        desc = bool(rng.integers(0, 2))
+        if u < 0.45:  # short qsorts (coalesced), a few type/flag combinations
+            tag = SMALL_TAGS[int(rng.integers(0, len(SMALL_TAGS)))]
+            n = int(rng.integers(0, 4097)) if tag != "u64" else int(rng.integers(0, 2049))
+            op, vtag, k = "qsort", None, 0
+        elif u < 0.65:
+            tag = list(T.TAGS)[int(rng.integers(0, 9))]
+            n = int(np.exp(rng.uniform(np.log(4097), np.log(1 << 18))))
+            op, vtag, k = "qsort", None, 0
+        elif u < 0.8:
+            tag = T.KV_TAGS[int(rng.integers(0, len(T.KV_TAGS)))]
+            vtag = KV_VAL[int(rng.integers(0, len(KV_VAL)))]
+            n = int(np.exp(rng.uniform(np.log(2), np.log(1 << 17))))
+            op, k = "keyvalue_qsort", 0
+        else:
+            tag = list(T.TAGS)[int(rng.integers(0, 9))]
+            n = int(np.exp(rng.uniform(np.log(2), np.log(1 << 18))))
+            op, vtag = ("qselect", None) if rng.random() < 0.5 else ("partial_qsort", None)
+            k = int(rng.integers(0, n)) if op == "qselect" else int(rng.integers(1, n + 1))
+        keys = T.gen(tag, dist, n, seed * 1000 + i)
+        hasnan = dist in T.NAN_DISTS and tag in T.FLOAT_TAGS
+        vals = None
+        if vtag is not None:
+            vals = rng.integers(0, 2 ** 31, n).astype(T.TAGS[vtag])
+        out.append((op, tag, vtag, keys, vals, k, hasnan, desc))
+    return out
+
+
+def _alone(xss, item):
+    op, tag, vtag, keys, vals, k, hasnan, desc = item
+    d = xss.DArray.from_numpy(keys)
+    if op == "qsort":
+        xss.qsort(d, hasnan, desc)
+        return d.numpy(), None
+    if op == "keyvalue_qsort":
+        v = xss.DArray.from_numpy(vals)
+        xss.keyvalue_qsort(d, v, hasnan, desc)
+        return d.numpy(), v.numpy()
+    if op == "qselect":
+        xss.qselect(d, k, hasnan, desc)
+    else:
+        xss.partial_qsort(d, k, hasnan, desc)
+    return d.numpy(), None
+
+
+def _serve(xss, mix, **server_args):
+    import torch
+    arrays = []
+    reqs = []
+    for op, tag, vtag, keys, vals, k, hasnan, desc in mix:
+        d = xss.DArray.from_numpy(keys)
+        v = xss.DArray.from_numpy(vals) if vals is not None else None
+        arrays.append((d, v))
+        reqs.append(xss.make_request(op, d, v, k, hasnan, desc))
+    torch.cuda.synchronize()  # inputs ready before the server's streams read them
+    with xss.Server(**server_args) as sv:
+        done = sv.run_trace(reqs)
+        stats = sv.stats()
+    assert all(r.latency_us >= 0 for r in done)
+    results = [(d.numpy(), v.numpy() if v is not None else None) for d, v in arrays]
+    return results, stats
+
+
+def _compare(mix, alone, served, what):
+    for i, (item, (ak, av), (sk, sv)) in enumerate(zip(mix, alone, served)):
+        op, tag, vtag, keys, vals, k, hasnan, desc = item
+        w = f"{what} request {i}: {op}<{tag}{',' + vtag if vtag else ''}> n={keys.size} k={k} " \
+            f"hasnan={hasnan} desc={desc}"
+        if op in ("qsort", "keyvalue_qsort"):
+            assert np.array_equal(T.bits(sk), T.bits(ak)), w + ": keys differ"
+            if op == "keyvalue_qsort":
+                assert np.array_equal(T.bits(sv), T.bits(av)), w + ": values differ"
+        elif op == "qselect":
+            assert np.array_equal(np.sort(T.bits(sk)), np.sort(T.bits(keys))), w + ": not a permutation"
+            cs, ca = T.vclass(sk, tag), T.vclass(ak, tag)
+            assert cs[k] == ca[k], w + ": k-th element differs"
+            assert np.array_equal(np.sort(cs[:k]), np.sort(ca[:k])), w + ": left partition differs"
+            assert np.array_equal(np.sort(cs[k + 1:]), np.sort(ca[k + 1:])), w + ": right partition differs"
+        else:
+            T.assert_prefix_equal(sk, ak, k, tag, w)
+            assert np.array_equal(np.sort(T.vclass(sk, tag)), np.sort(T.vclass(ak, tag))), w
+
+
+@pytest.fixture(scope="module")
+def mix_and_alone(xss):
+    mix = _mix(240, 31)
+    return mix, [_alone(xss, it) for it in mix]
+
+
+def test_serving_burst_coalesced(xss, mix_and_alone):
+    mix, alone = mix_and_alone
+    served, stats = _serve(xss, mix, budget_bytes=64 << 30, streams=4, coalesce_max=4096)
+    _compare(mix, alone, served, "burst")
+    assert stats["completed"] == len(mix)
+    assert stats["coalesced"] > 0, stats  # short qsorts queued together shared a launch
+    assert stats["launch_groups"] < len(mix)
+
+
+def test_serving_tight_budget(xss, mix_and_alone):
+    mix, alone = mix_and_alone
+    # 8 MiB admits a few requests at a time; larger ones run alone when nothing else is in flight
+    served, stats = _serve(xss, mix, budget_bytes=8 << 20, streams=2, coalesce_max=1024)
+    _compare(mix, alone, served, "tight budget")
+    assert stats["completed"] == len(mix)
+
+
+def test_serving_one_at_a_time(xss, mix_and_alone):
+    mix, alone = mix_and_alone
+    served, stats = _serve(xss, mix, budget_bytes=64 << 30, streams=1, coalesce_max=0, one_at_a_time=True)
+    _compare(mix, alone, served, "one at a time")
+    assert stats["launch_groups"] == len(mix) and stats["coalesced"] == 0
+
+
+def test_serving_poisson_open_loop(xss):
+    """Requests submitted at Poisson arrival times while earlier ones run."""
+    import torch
+    mix = _mix(60, 77)
+    alone = [_alone(xss, it) for it in mix]
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(200.0, len(mix)))  # 5k req/s
+    arrays, reqs = [], []
+    for (op, tag, vtag, keys, vals, k, hasnan, desc), t in zip(mix, arrivals):
+        d = xss.DArray.from_numpy(keys)
+        v = xss.DArray.from_numpy(vals) if vals is not None else None
+        arrays.append((d, v))
+        reqs.append(xss.make_request(op, d, v, k, hasnan, desc, arrival_us=t))
+    torch.cuda.synchronize()
+    with xss.Server(streams=4) as sv:
+        done = sv.run_trace(reqs)
+    lat = np.array([r.latency_us for r in done])
+    assert (lat >= 0).all()
+    served = [(d.numpy(), v.numpy() if v is not None else None) for d, v in arrays]
+    _compare(mix, alone, served, "poisson")

@@ -411,6 +411,9 @@ struct BlockSortArgs {
     void *vals_out;
     uint64_t *idx_out;         // PL_IDX
     uint32_t *idx32_out;       // PL_IDX: segment-local 32-bit indices instead
+    const uint64_t *arr_keys;  // pointer-list form: array s at arr_keys[s] (values at
+    const uint64_t *arr_vals;  // arr_vals[s]) with arr_len[s] keys, sorted in place;
+    const uint32_t *arr_len;   // null: the segment forms below
     const uint64_t *seg_off;   // segment s = [seg_off[s], seg_off[s+1]); null: one array [0, n)
     const uint32_t *seg_list;  // segment ids handled by this launch (null: blockIdx.x)
     uint32_t n;
@@ -435,8 +438,13 @@ __global__ __launch_bounds__(BLOCK) void block_sort_kernel(BlockSortArgs a)
     __shared__ V s_val[HAS_VAL ? MAXN : 1];
 
     const uint32_t seg = a.seg_list ? a.seg_list[blockIdx.x] : blockIdx.x;
+    if (a.arr_keys) {
+        // pointer-list form: rebase this workgroup onto array `seg`
+        a.keys_in = a.keys_out = reinterpret_cast<void *>(a.arr_keys[seg]);
+        if (a.arr_vals) a.vals_in = a.vals_out = reinterpret_cast<void *>(a.arr_vals[seg]);
+    }
     const uint64_t base = a.seg_off ? a.seg_off[seg] : 0;
-    const uint32_t n = a.seg_off ? uint32_t(a.seg_off[seg + 1] - base) : a.n;
+    const uint32_t n = a.arr_keys ? a.arr_len[seg] : a.seg_off ? uint32_t(a.seg_off[seg + 1] - base) : a.n;
     if (n < 2) {
         if (n == 1) {
             const U *kin = static_cast<const U *>(a.keys_in);

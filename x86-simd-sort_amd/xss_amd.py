@@ -143,3 +143,76 @@ def object_sort_by_keys(objs, obj_bytes, keys: DArray):
     _check(getattr(lib(), f"xss_gpu_object_sort_by_keys_{keys.tag}")(
         ctypes.c_void_p(objs.data_ptr()), _sz(obj_bytes), keys.ptr, _sz(keys.n), _stream()),
         f"object_sort_by_keys<{keys.tag}>")
+
+
+# ---------------------------------------------------------------- serving form
+
+class Request(ctypes.Structure):
+    """struct xss_request of include/xss_gpu.h."""
+    _fields_ = [("op", ctypes.c_int), ("key_type", ctypes.c_int), ("val_type", ctypes.c_int),
+                ("hasnan", ctypes.c_int), ("descending", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("keys", ctypes.c_void_p), ("vals", ctypes.c_void_p), ("n", ctypes.c_uint64),
+                ("k", ctypes.c_uint64), ("arrival_us", ctypes.c_double), ("latency_us", ctypes.c_double),
+                ("internal", ctypes.c_void_p)]
+
+
+SERVE_OPS = {"qsort": 0, "keyvalue_qsort": 1, "qselect": 2, "partial_qsort": 3}
+TYPE_CODE = {tag: i for i, tag in enumerate(TAGS)}  # u16 .. f64 -> 0 .. 8
+
+
+def make_request(op, keys: DArray, vals: DArray = None, k=0, hasnan=False, descending=False, arrival_us=0.0):
+    """A request on device arrays.  k is qselect's 0-based rank or
+    partial_qsort's count, as in the synchronous calls."""
+    r = Request()
+    r.op = SERVE_OPS[op]
+    r.key_type = TYPE_CODE[keys.tag]
+    r.val_type = TYPE_CODE[vals.tag] if vals is not None else 0
+    r.hasnan, r.descending = int(hasnan), int(descending)
+    r.keys = keys.t.data_ptr()
+    r.vals = vals.t.data_ptr() if vals is not None else None
+    r.n, r.k, r.arrival_us = keys.n, int(k), float(arrival_us)
+    return r
+
+
+class Server:
+    """Continuous-batching request loop (xss_server_* of include/xss_gpu.h).
+
+    Requests run on the server's own streams: the caller makes their inputs
+    ready (e.g. torch.cuda.synchronize()) before submitting them."""
+
+    def __init__(self, budget_bytes=64 << 30, streams=4, coalesce_max=4096, reserve_bytes=0, one_at_a_time=False):
+        L = lib()
+        self.h = ctypes.c_void_p()
+        _check(L.xss_server_create(ctypes.byref(self.h), _sz(budget_bytes), int(streams),
+                                   ctypes.c_uint32(coalesce_max), _sz(reserve_bytes), int(one_at_a_time)),
+               "xss_server_create")
+
+    def submit(self, req: Request):
+        _check(lib().xss_server_submit(self.h, ctypes.byref(req)), "xss_server_submit")
+
+    def wait(self, req: Request):
+        _check(lib().xss_server_wait(self.h, ctypes.byref(req)), "xss_server_wait")
+
+    def run_trace(self, reqs):
+        """Submit every request at its arrival_us (open loop) and wait for all;
+        returns the request array with latency_us filled in."""
+        arr = reqs if isinstance(reqs, ctypes.Array) else (Request * len(reqs))(*reqs)
+        _check(lib().xss_server_run_trace(self.h, arr, _sz(len(reqs))), "xss_server_run_trace")
+        return arr
+
+    def stats(self):
+        s = (ctypes.c_uint64 * 3)()
+        _check(lib().xss_server_stats(self.h, ctypes.byref(s, 0), ctypes.byref(s, 8), ctypes.byref(s, 16)),
+               "xss_server_stats")
+        return {"completed": s[0], "launch_groups": s[1], "coalesced": s[2]}
+
+    def close(self):
+        if self.h:
+            lib().xss_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -2,6 +2,12 @@
 
     python x86-simd-sort_amd/build.py              # full rebuild
     python x86-simd-sort_amd/build.py --incremental
+    python x86-simd-sort_amd/build.py --out DIR -D XSS_PASS_ITEMS_C1=12   # a tuning variant
+
+Tuning knobs (-D, compile-time): XSS_PASS_BLOCK_Cn, XSS_PASS_ITEMS_Cn and
+XSS_SCATTER_WAVES_Cn (n = 0..3, the item-width classes of xss_kernels.hpp)
+set the tile shape and register budget of the scatter passes.  --out builds into DIR/_build and DIR/libxss_amd.so
+instead of the package directory.
 
 Every translation unit under csrc/ is compiled with hipcc for gfx950 into
 _build/, then linked into x86-simd-sort_amd/libxss_amd.so (in-tree, so the
@@ -27,6 +33,7 @@ COMMON = [
     "-ffp-contract=off",
     "-Wno-unused-result",
 ]
+EXTRA = []  # -D tuning knobs
 
 
 def sources():
@@ -49,7 +56,7 @@ def obj_of(src):
 
 def compile_one(src):
     obj = obj_of(src)
-    cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", src, "-o", obj]
+    cmd = [HIPCC] + COMMON + EXTRA + ["-x", "hip", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -83,5 +90,11 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--incremental", action="store_true")
     ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    ap.add_argument("--out", default=None)
     a = ap.parse_args()
+    EXTRA[:] = [f"-D{d}" for d in a.defines]
+    if a.out:
+        BUILD = os.path.join(os.path.abspath(a.out), "_build")
+        LIB = os.path.join(os.path.abspath(a.out), "libxss_amd.so")
     print(build(a.incremental, a.j))

@@ -380,8 +380,66 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
     }
 }
 
+// Tile shape and register budget of the scatter passes, per class of item
+// width (key + payload bytes): C0 <= 2 (16-bit keys), C1 = 4 (32-bit keys),
+// C2 = 5..8 (64-bit keys, or a 16/32-bit key with a 32-bit payload or
+// index), C3 > 8.  Build-time knobs; the defaults are the measured best on
+// MI355X.  A workgroup's waves must fit the register file: 1024 threads
+// leave 128 VGPRs per lane, 512 threads at most 256 (XSS_SCATTER_WAVES_Cn
+// caps the budget at 512 / waves VGPRs; 1 = the compiler's choice).  Larger
+// tiles write longer runs per digit, so fewer partially written lines.
+#ifndef XSS_PASS_BLOCK_C0
+#define XSS_PASS_BLOCK_C0 512
+#endif
+#ifndef XSS_PASS_ITEMS_C0
+#define XSS_PASS_ITEMS_C0 16
+#endif
+#ifndef XSS_SCATTER_WAVES_C0
+#define XSS_SCATTER_WAVES_C0 1
+#endif
+#ifndef XSS_PASS_BLOCK_C1
+#define XSS_PASS_BLOCK_C1 1024
+#endif
+#ifndef XSS_PASS_ITEMS_C1
+#define XSS_PASS_ITEMS_C1 16
+#endif
+#ifndef XSS_SCATTER_WAVES_C1
+#define XSS_SCATTER_WAVES_C1 1
+#endif
+#ifndef XSS_PASS_BLOCK_C2
+#define XSS_PASS_BLOCK_C2 1024
+#endif
+#ifndef XSS_PASS_ITEMS_C2
+#define XSS_PASS_ITEMS_C2 12
+#endif
+#ifndef XSS_SCATTER_WAVES_C2
+#define XSS_SCATTER_WAVES_C2 1
+#endif
+#ifndef XSS_PASS_BLOCK_C3
+#define XSS_PASS_BLOCK_C3 1024
+#endif
+#ifndef XSS_PASS_ITEMS_C3
+#define XSS_PASS_ITEMS_C3 8
+#endif
+#ifndef XSS_SCATTER_WAVES_C3
+#define XSS_SCATTER_WAVES_C3 1
+#endif
+template <class KD, int PL>
+struct ScatterClass {
+    static constexpr int bytes =
+        int(sizeof(typename KD::U)) + (PL == PL_NONE ? 0 : int(sizeof(typename PayloadT<PL>::type)));
+    static constexpr int value = bytes <= 2 ? 0 : bytes <= 4 ? 1 : bytes <= 8 ? 2 : 3;
+};
+template <class KD, int PL>
+struct ScatterWaves {
+    static constexpr int c = ScatterClass<KD, PL>::value;
+    static constexpr int value = c == 0 ? XSS_SCATTER_WAVES_C0 : c == 1 ? XSS_SCATTER_WAVES_C1
+                               : c == 2 ? XSS_SCATTER_WAVES_C2 : XSS_SCATTER_WAVES_C3;
+};
+#define XSS_SCATTER_OCCUPANCY(KD, PL) __attribute__((amdgpu_waves_per_eu(ScatterWaves<KD, PL>::value)))
+
 template <class KD, int PL, int BLOCK, int ITEMS, bool NL>
-__global__ __launch_bounds__(BLOCK) void downsweep_kernel(PassArgs a)
+__global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL) void downsweep_kernel(PassArgs a)
 {
     constexpr int TILE = BLOCK * ITEMS;
     static_assert(BLOCK >= 256 && BLOCK % 64 == 0, "one thread per digit needed");

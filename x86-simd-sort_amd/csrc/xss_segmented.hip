@@ -91,7 +91,7 @@ struct SegRanges {
 };
 
 template <class KD, int PL, int BLOCK, int ITEMS, bool NL>
-__global__ __launch_bounds__(BLOCK) void seg_downsweep_kernel(PassArgs a, SegRanges r)
+__global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL) void seg_downsweep_kernel(PassArgs a, SegRanges r)
 {
     constexpr int TILE = BLOCK * ITEMS;
     __shared__ DownsweepSmem<KD, PL, BLOCK, ITEMS> sm;

@@ -41,8 +41,11 @@ template <class KD, int PL>
 struct PassShape {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
-    static constexpr int BLOCK = 512;
-    static constexpr int ITEMS = (sizeof(U) + (PL == PL_NONE ? 0 : sizeof(V))) > 8 ? 8 : 16;
+    static constexpr int CLS = ScatterClass<KD, PL>::value;
+    static constexpr int BLOCK = CLS == 0 ? XSS_PASS_BLOCK_C0 : CLS == 1 ? XSS_PASS_BLOCK_C1
+                               : CLS == 2 ? XSS_PASS_BLOCK_C2 : XSS_PASS_BLOCK_C3;
+    static constexpr int ITEMS = CLS == 0 ? XSS_PASS_ITEMS_C0 : CLS == 1 ? XSS_PASS_ITEMS_C1
+                               : CLS == 2 ? XSS_PASS_ITEMS_C2 : XSS_PASS_ITEMS_C3;
     static constexpr int TILE = BLOCK * ITEMS;
 };
 

@@ -50,7 +50,7 @@ def main():
         best = None
         rows = []
         for small_max in [512, 1024, 2048, 4096]:
-            for range_tiles in [2, 4, 8, 16]:
+            for range_tiles in [1, 2, 4, 8]:
                 p = ctypes.c_void_p()
                 rc = L.xss_gpu_segplan_create(ctypes.byref(p), op.encode(), tag.encode(), offp,
                                               ctypes.c_size_t(args.segments), ctypes.c_uint32(small_max),

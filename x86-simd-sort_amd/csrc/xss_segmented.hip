@@ -90,8 +90,22 @@ struct SegRanges {
     uint32_t nr;
 };
 
+// Tile shape of the segmented passes: 512 threads x 16 items (8 for items
+// wider than 8 bytes), smaller than the one-array pass for 4-byte keys
+// because a segment of a few tiles wastes less of a smaller last tile.  The
+// host-side plan (shape_of below) cuts ranges with the same tile size.
+// Keys-only 4-byte tiles get a 128-VGPR budget (two workgroups per CU).
+template <class KD, int PL>
+struct SegShape {
+    static constexpr int BYTES = ScatterClass<KD, PL>::bytes;
+    static constexpr int BLOCK = 512;
+    static constexpr int ITEMS = BYTES > 8 ? 8 : 16;
+    static constexpr int WAVES = BYTES == 4 ? 4 : 1;
+};
+
 template <class KD, int PL, int BLOCK, int ITEMS, bool NL>
-__global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL) void seg_downsweep_kernel(PassArgs a, SegRanges r)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SegShape<KD, PL>::WAVES)))
+void seg_downsweep_kernel(PassArgs a, SegRanges r)
 {
     constexpr int TILE = BLOCK * ITEMS;
     __shared__ DownsweepSmem<KD, PL, BLOCK, ITEMS> sm;
@@ -144,7 +158,7 @@ template <class KD, int PL>
 int seg_run(xss_segplan *p, void *keys, uint32_t *idx32, bool hasnan, bool desc, hipStream_t s)
 {
     using U = typename KD::U;
-    using PS = PassShape<KD, PL>;
+    using PS = SegShape<KD, PL>;
     using BS = BlockShape<KD, PL>;
     constexpr int P = KD::PASSES;
     const uint64_t flip = desc ? ~0ull : 0ull;
@@ -238,7 +252,7 @@ void shape_of(const std::string &tag, int op, uint32_t *maxn, uint32_t *tile)
 {
     const int kb = key_bytes(tag);
     const bool idx = op == 1;
-    // BlockShape / PassShape of xss_sort.hpp for keys-only and index payloads
+    // BlockShape of xss_sort.hpp and SegShape above, for keys-only and index payloads
     const int bpe = 2 * kb + 2;
     *maxn = (65536 / bpe) >= 4096 ? 4096 : (65536 / bpe) >= 2048 ? 2048 : 1024;
     const int items = (kb + (idx ? 4 : 0)) > 8 ? 8 : 16;

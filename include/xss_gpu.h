@@ -66,6 +66,11 @@ XSS_API size_t xss_gpu_workspace_bytes(void);
  * waits for them, returns up to max_names (name, total ms, launches) rows
  * and clears the record.  No counterpart in the reference. */
 XSS_API int xss_gpu_timing_enable(int on);
+/* Runtime tuning switches (no counterpart in the reference):
+ *   "hybrid_min_n"  qsort of 4- and 8-byte keys sorts arrays of at least
+ *                   this many keys MSD-first (two passes over HBM, then every
+ *                   bucket of the top 16 bits in LDS); default 2^25. */
+XSS_API int xss_gpu_set_tuning(const char *name, uint64_t value);
 XSS_API int xss_gpu_timing_collect(int max_names, const char **names, double *total_ms, uint64_t *counts);
 
 #define XSS_DECLARE_T(tag, T)                                                                      \

@@ -1,0 +1,451 @@
+// xss_hybrid.hpp -- MSD-first sort of one large keys-only array.
+//
+// An LSD sort of 4-byte keys streams the array through HBM four times (each
+// pass: a histogram read, then a read and a write).  For large arrays this
+// path does two passes and then finishes every bucket on chip:
+//
+//   pass A   LSD pass on the top digit over the whole array (keys -> scratch)
+//   table    the 256 top-digit buckets are cut into ranges (built on the
+//            device from pass A's digit totals)
+//   pass B   segmented pass on the second digit inside every top bucket
+//            (scratch -> keys); its per-bucket digit totals are the exact
+//            sizes of the 65536 buckets of the top two digits
+//   buckets  bucket starts; buckets larger than one workgroup's LDS are listed
+//   local    one workgroup per bucket sorts it by the remaining low digits in
+//            LDS (an 8-bit LSD radix per digit, stable) and writes it back
+//   large    listed buckets (skewed or duplicate-heavy data) get segmented
+//            LSD passes on the low digits (keys -> scratch -> keys)
+//
+// Everything is enqueued on the stream without a host round trip (the range
+// tables are built by kernels and passes over them are launched at the
+// tables' capacity, surplus workgroups exiting), so the call stays
+// asynchronous and capturable.  Keys-only sorts need no stability, but every
+// step is stable anyway.
+#pragma once
+#include <atomic>
+
+#include "xss_sort.hpp"
+
+namespace xss {
+
+constexpr int HYB_BLOCK = 512;
+constexpr int HYB_ITEMS = 12;
+constexpr uint32_t HYB_CAP = HYB_BLOCK * HYB_ITEMS;  // largest bucket sorted in LDS
+constexpr uint32_t HYB_BUCKETS = 65536;              // buckets of the top two digits
+constexpr size_t HYB_MIN_N = size_t(1) << 25;        // smaller arrays take the LSD path
+
+struct HybTable {        // a device-built range table of segmented passes
+    uint64_t *begin;
+    uint64_t *end;
+    uint64_t *sbase;     // first key of the range's segment
+    uint32_t *bseg;      // segment index
+    uint32_t *head;      // 1 at a segment's first range
+    uint32_t *nr;        // ranges built (device)
+    uint32_t cap;        // capacity = grid of the passes over the table
+};
+
+struct HybArgs {
+    const uint32_t *top_total;  // [256] key counts per top digit (pass A)
+    uint64_t *seg_off;          // [257] top-bucket starts
+    const uint32_t *segtot;     // [256][256] second-digit counts per top bucket (pass B)
+    uint32_t *bstart;           // [65536] bucket starts
+    uint32_t *bsize;            // [65536] bucket sizes
+    uint32_t *nlarge;           // buckets larger than HYB_CAP (device)
+    uint32_t *lstart;           // [lcap] the large buckets
+    uint32_t *lsize;
+    uint32_t lcap;
+    HybTable tb;                // ranges of pass B (segments: top buckets)
+    HybTable td;                // ranges of the low passes (segments: large buckets)
+    uint64_t rk_b;              // keys per range
+    uint64_t rk_d;
+};
+
+// ranges of pass B from pass A's digit totals (one workgroup)
+template <int = 0>
+__global__ __launch_bounds__(256) void hyb_top_ranges_kernel(HybArgs a)
+{
+    __shared__ uint32_t w1[4], w2[4];
+    const uint32_t s = threadIdx.x, lane = s & 63, w = s >> 6;
+    const uint32_t size = a.top_total[s];
+    const uint32_t nrs = uint32_t((size + a.rk_b - 1) / a.rk_b);
+    uint32_t i1 = size, i2 = nrs;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t1 = __shfl_up(i1, off), t2 = __shfl_up(i2, off);
+        if (lane >= uint32_t(off)) {
+            i1 += t1;
+            i2 += t2;
+        }
+    }
+    if (lane == 63) {
+        w1[w] = i1;
+        w2[w] = i2;
+    }
+    __syncthreads();
+    uint32_t p1 = 0, p2 = 0;
+    for (uint32_t k = 0; k < w; ++k) {
+        p1 += w1[k];
+        p2 += w2[k];
+    }
+    const uint64_t start = p1 + i1 - size;
+    const uint32_t r0 = p2 + i2 - nrs;
+    a.seg_off[s] = start;
+    if (s == 255) {
+        a.seg_off[256] = start + size;
+        *a.tb.nr = r0 + nrs;
+        *a.nlarge = 0;
+    }
+    for (uint32_t j = 0; j < nrs; ++j) {
+        const uint32_t r = r0 + j;
+        if (r >= a.tb.cap) break;  // cannot happen: cap = ceil(n / rk) + 256
+        const uint64_t b = start + uint64_t(j) * a.rk_b;
+        a.tb.begin[r] = b;
+        a.tb.end[r] = min(start + size, b + a.rk_b);
+        a.tb.sbase[r] = start;
+        a.tb.bseg[r] = s;
+        a.tb.head[r] = j == 0 ? 1u : 0u;
+    }
+}
+
+// bucket starts / sizes of the top two digits; list the large ones
+// (workgroup = top bucket, thread = second digit)
+template <int = 0>
+__global__ __launch_bounds__(256) void hyb_buckets_kernel(HybArgs a)
+{
+    __shared__ uint32_t wsum[4];
+    const uint32_t s = blockIdx.x, d = threadIdx.x, lane = d & 63, w = d >> 6;
+    const uint64_t s0 = a.seg_off[s], s1 = a.seg_off[s + 1];
+    // an empty top bucket has no ranges in pass B, so no totals were written for it
+    const uint32_t size = s1 > s0 ? a.segtot[size_t(s) * 256 + d] : 0u;
+    uint32_t incl = size;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off);
+        if (lane >= uint32_t(off)) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (uint32_t k = 0; k < w; ++k) pre += wsum[k];
+    const uint32_t start = uint32_t(s0) + pre + incl - size;
+    const uint32_t b = s * 256 + d;
+    a.bstart[b] = start;
+    a.bsize[b] = size;
+    if (size > HYB_CAP) {
+        const uint32_t i = atomicAdd(a.nlarge, 1u);
+        if (i < a.lcap) {  // lcap >= n / (HYB_CAP + 1): always true
+            a.lstart[i] = start;
+            a.lsize[i] = size;
+        }
+    }
+}
+
+// ranges of the low passes over the large buckets (one workgroup)
+template <int = 0>
+__global__ __launch_bounds__(1024) void hyb_large_ranges_kernel(HybArgs a)
+{
+    __shared__ uint32_t wsum[16];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t nl = min(*a.nlarge, a.lcap);
+    const uint32_t per = (nl + 1023) / 1024;
+    const uint32_t c0 = min(nl, t * per), c1 = min(nl, c0 + per);
+    uint32_t cnt = 0;
+    for (uint32_t i = c0; i < c1; ++i) cnt += uint32_t((a.lsize[i] + a.rk_d - 1) / a.rk_d);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t x = __shfl_up(incl, off);
+        if (lane >= uint32_t(off)) incl += x;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t r = incl - cnt;
+    for (uint32_t k = 0; k < w; ++k) r += wsum[k];
+    if (t == 1023) *a.td.nr = min(a.td.cap, r + cnt);
+    for (uint32_t i = c0; i < c1; ++i) {
+        const uint64_t st = a.lstart[i], sz = a.lsize[i];
+        for (uint64_t x = st; x < st + sz; x += a.rk_d, ++r) {
+            if (r >= a.td.cap) return;  // cannot happen: cap = ceil(n / rk) + lcap
+            a.td.begin[r] = x;
+            a.td.end[r] = min(st + sz, x + a.rk_d);
+            a.td.sbase[r] = st;
+            a.td.bseg[r] = i;
+            a.td.head[r] = x == st ? 1u : 0u;
+        }
+    }
+}
+
+// one workgroup per bucket of at most HYB_CAP keys: LSD radix by the low
+// P - 2 digits in LDS (the top two are the bucket's), written back in place
+template <class KD, bool NL>
+__global__ __launch_bounds__(HYB_BLOCK) void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize,
+                                                              void *keys_v, uint64_t flip64, uint32_t canon)
+{
+    using U = typename KD::U;
+    constexpr int P = KD::PASSES;
+    constexpr int WAVES = HYB_BLOCK / 64;
+    constexpr int WT = 64 * HYB_ITEMS;
+    __shared__ U s_keys[HYB_CAP];
+    __shared__ uint32_t s_off[WAVES * 256];
+    __shared__ uint32_t s_wsum[WAVES];
+    const uint32_t b = blockIdx.x;
+    const uint32_t m = bsize[b];
+    if (m > HYB_CAP) return;  // a large bucket: the low segmented passes sort it
+    U *keys = static_cast<U *>(keys_v) + bstart[b];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    auto finish = [&](U x) {
+        if constexpr (KD::KIND == Kind::Float)
+            if (canon && KD::is_nan(x)) x = KD::QNAN;
+        return x;
+    };
+    if (m < 2) {
+        if (m == 1 && tid == 0) keys[0] = finish(keys[0]);
+        return;
+    }
+    const U flip = U(flip64);
+    const uint32_t lbase = wave * WT + lane;
+    U key[HYB_ITEMS];
+#pragma unroll
+    for (int i = 0; i < HYB_ITEMS; ++i) {
+        const uint32_t idx = lbase + i * 64;
+        key[i] = idx < m ? keys[idx] : U(0);
+    }
+#pragma unroll 1
+    for (int p = 0; p < P - 2; ++p) {
+        const int shift = 8 * p;
+        for (uint32_t i = tid; i < WAVES * 256; i += HYB_BLOCK) s_off[i] = 0;
+        __syncthreads();  // counters cleared; the previous pass's reads of s_keys are done
+        // rank: stable within the bucket (wave-striped order), per-wave counters;
+        // padding slots take digit 255 and come after every key
+        uint32_t rd[HYB_ITEMS];
+        uint32_t *wc = s_off + wave * 256;
+#pragma unroll
+        for (int i = 0; i < HYB_ITEMS; ++i) {
+            const uint32_t idx = lbase + i * 64;
+            const uint32_t d = idx < m ? KD::digit(key[i], flip, NL, shift) : 255u;
+            const uint64_t peers = match_any8(d);
+            const uint32_t below = lanes_below(peers);
+            const uint32_t tot = uint32_t(__popcll(peers));
+            const uint32_t c = wc[d];
+            rd[i] = (c + below) | (d << 16);
+            if (below + 1 == tot) wc[d] = c + tot;
+        }
+        __syncthreads();
+        // digit counts, their exclusive scan, and the slot base of every (wave, digit)
+        uint32_t tcount = 0;
+        if (tid < 256) {
+#pragma unroll
+            for (int w = 0; w < WAVES; ++w) tcount += s_off[w * 256 + tid];
+        }
+        uint32_t incl = tcount;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off);
+            if (lane >= uint32_t(off)) incl += t;
+        }
+        if (lane == 63) s_wsum[wave] = incl;
+        __syncthreads();
+        if (tid < 256) {
+            uint32_t st = incl - tcount;
+            for (uint32_t w = 0; w < wave; ++w) st += s_wsum[w];
+#pragma unroll
+            for (int w = 0; w < WAVES; ++w) {
+                const uint32_t c = s_off[w * 256 + tid];
+                s_off[w * 256 + tid] = st;
+                st += c;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < HYB_ITEMS; ++i) {
+            const uint32_t d = rd[i] >> 16;
+            s_keys[wc[d] + (rd[i] & 0xffffu)] = key[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < HYB_ITEMS; ++i) key[i] = s_keys[lbase + i * 64];
+    }
+#pragma unroll
+    for (int i = 0; i < HYB_ITEMS; ++i) {
+        const uint32_t idx = lbase + i * 64;
+        if (idx < m) keys[idx] = finish(key[i]);
+    }
+}
+
+template <class KD>
+struct HybPlan {
+    using U = typename KD::U;
+    using SS = SegShape<KD, PL_NONE>;
+    static constexpr uint64_t SEG_TILE = uint64_t(SS::BLOCK) * SS::ITEMS;
+    // pass B's table must fit the workspace's [256][MAX_RANGES] range tables
+    static uint64_t rk_b(size_t n)
+    {
+        const uint64_t per = uint64_t(MAX_RANGES - 256) * SEG_TILE;
+        return SEG_TILE * std::max<uint64_t>(1, (n + per - 1) / per);
+    }
+    static uint32_t cap_b(size_t n) { return uint32_t((n + rk_b(n) - 1) / rk_b(n) + 256); }
+    static uint64_t rk_d(size_t) { return 8 * SEG_TILE; }
+    static uint32_t lcap(size_t n) { return uint32_t(std::min<size_t>(HYB_BUCKETS, n / (HYB_CAP + 1) + 1)); }
+    static uint32_t cap_d(size_t n) { return uint32_t((n + rk_d(n) - 1) / rk_d(n) + lcap(n)); }
+    static size_t table_bytes(uint32_t cap) { return 3 * align256(size_t(cap) * 8) + 2 * align256(size_t(cap) * 4); }
+    static size_t scratch_bytes(size_t n)
+    {
+        return align256(n * sizeof(U)) + align256(256 * 256 * 4) + align256(257 * 8) + 2 * align256(HYB_BUCKETS * 4) +
+               align256(16) + table_bytes(cap_b(n)) + 2 * align256(size_t(lcap(n)) * 4) + table_bytes(cap_d(n)) +
+               2 * align256(size_t(256) * cap_d(n) * 4) + align256(size_t(lcap(n)) * 256 * 4);
+    }
+};
+
+template <class KD>
+int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasnan, bool desc, hipStream_t s)
+{
+    using U = typename KD::U;
+    using Plan = SortPlan<KD, PL_NONE>;
+    using PS = PassShape<KD, PL_NONE>;
+    using SS = SegShape<KD, PL_NONE>;
+    using HP = HybPlan<KD>;
+    constexpr int P = KD::PASSES;
+    static_assert(P >= 4, "two top digits and at least two low digits");
+    const uint64_t flip = desc ? ~0ull : 0ull;
+    const bool nan_last = (KD::KIND == Kind::Float) && hasnan;
+    const uint32_t nn = uint32_t(n);
+    const uint32_t cap_b = HP::cap_b(n), cap_d = HP::cap_d(n), lcap = HP::lcap(n);
+
+    // scratch layout (HybPlan::scratch_bytes)
+    char *p = scratch;
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += align256(bytes);
+        return q;
+    };
+    char *k0 = take(n * sizeof(U));
+    uint32_t *segtot = reinterpret_cast<uint32_t *>(take(256 * 256 * 4));
+    HybArgs h{};
+    h.seg_off = reinterpret_cast<uint64_t *>(take(257 * 8));
+    h.bstart = reinterpret_cast<uint32_t *>(take(HYB_BUCKETS * 4));
+    h.bsize = reinterpret_cast<uint32_t *>(take(HYB_BUCKETS * 4));
+    uint32_t *ctr = reinterpret_cast<uint32_t *>(take(16));
+    auto table = [&](HybTable &t, uint32_t cap, uint32_t *nr) {
+        t.begin = reinterpret_cast<uint64_t *>(take(size_t(cap) * 8));
+        t.end = reinterpret_cast<uint64_t *>(take(size_t(cap) * 8));
+        t.sbase = reinterpret_cast<uint64_t *>(take(size_t(cap) * 8));
+        t.bseg = reinterpret_cast<uint32_t *>(take(size_t(cap) * 4));
+        t.head = reinterpret_cast<uint32_t *>(take(size_t(cap) * 4));
+        t.nr = nr;
+        t.cap = cap;
+    };
+    table(h.tb, cap_b, ctr + 1);
+    h.lstart = reinterpret_cast<uint32_t *>(take(size_t(lcap) * 4));
+    h.lsize = reinterpret_cast<uint32_t *>(take(size_t(lcap) * 4));
+    table(h.td, cap_d, ctr + 2);
+    uint32_t *rsum_d = reinterpret_cast<uint32_t *>(take(size_t(256) * cap_d * 4));
+    uint32_t *rpre_d = reinterpret_cast<uint32_t *>(take(size_t(256) * cap_d * 4));
+    uint32_t *segtot_d = reinterpret_cast<uint32_t *>(take(size_t(lcap) * 256 * 4));
+    h.top_total = ws->total();
+    h.segtot = segtot;
+    h.nlarge = ctr;
+    h.lcap = lcap;
+    h.rk_b = HP::rk_b(n);
+    h.rk_d = HP::rk_d(n);
+
+    KernelTimer timer("hybrid_sort", s);
+    // pass A: top digit over the whole array (keys -> k0)
+    {
+        const uint32_t rtiles = Plan::range_tiles(n), ranges = Plan::ranges(n);
+        UpArgs u{};
+        u.keys = keys; u.n = nn; u.range_keys = rtiles * PS::TILE; u.shift = 8 * (P - 1);
+        u.nan_last = nan_last; u.flip = flip; u.rsum = ws->rsum(); u.ranges = ranges;
+        hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
+        XSS_HIP_CHECK(hipGetLastError());
+        ScanArgs sc{};
+        sc.rsum = ws->rsum(); sc.rpre = ws->rpre(); sc.total = ws->total(); sc.ranges = ranges;
+        hipLaunchKernelGGL(range_scan_kernel<0>, dim3(256), dim3(SCAN_BLOCK), 0, s, sc);
+        XSS_HIP_CHECK(hipGetLastError());
+        PassArgs a{};
+        a.keys_in = keys; a.keys_out = k0; a.n = nn; a.shift = 8 * (P - 1); a.flip = flip;
+        a.nan_last = nan_last; a.range_tiles = rtiles; a.ranges = ranges; a.rpre = ws->rpre(); a.total = ws->total();
+        if (nan_last)
+            hipLaunchKernelGGL((downsweep_kernel<KD, PL_NONE, PS::BLOCK, PS::ITEMS, true>), dim3(ranges),
+                               dim3(PS::BLOCK), 0, s, a);
+        else
+            hipLaunchKernelGGL((downsweep_kernel<KD, PL_NONE, PS::BLOCK, PS::ITEMS, false>), dim3(ranges),
+                               dim3(PS::BLOCK), 0, s, a);
+        XSS_HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(hyb_top_ranges_kernel<0>, dim3(1), dim3(256), 0, s, h);
+    XSS_HIP_CHECK(hipGetLastError());
+    // one segmented pass over a device-built table (src -> dst)
+    auto seg_pass = [&](const HybTable &t, uint32_t *rsum, uint32_t *rpre, uint32_t *stot, int shift, const void *src,
+                        void *dst, bool canon) -> int {
+        UpArgs u{};
+        u.keys = src; u.n = nn; u.shift = uint32_t(shift); u.nan_last = nan_last; u.flip = flip;
+        u.rsum = rsum; u.ranges = t.cap; u.range_begin = t.begin; u.range_end = t.end; u.nr_dev = t.nr;
+        hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(t.cap), dim3(UP_BLOCK), 0, s, u);
+        XSS_HIP_CHECK(hipGetLastError());
+        SegScanArgs sc{rsum, rpre, stot, t.head, t.bseg, 0, t.cap, t.nr};
+        hipLaunchKernelGGL(seg_scan_kernel<0>, dim3(256), dim3(256), 0, s, sc);
+        XSS_HIP_CHECK(hipGetLastError());
+        PassArgs a{};
+        a.keys_in = src; a.keys_out = dst; a.n = nn; a.shift = uint32_t(shift); a.flip = flip;
+        a.nan_last = nan_last; a.canon_nan = canon && nan_last;
+        SegRanges r{t.begin, t.end, t.sbase, t.bseg, rpre, stot, 0, t.cap, t.nr};
+        if (nan_last)
+            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL_NONE, SS::BLOCK, SS::ITEMS, true>), dim3(t.cap),
+                               dim3(SS::BLOCK), 0, s, a, r);
+        else
+            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL_NONE, SS::BLOCK, SS::ITEMS, false>), dim3(t.cap),
+                               dim3(SS::BLOCK), 0, s, a, r);
+        XSS_HIP_CHECK(hipGetLastError());
+        return XSS_OK;
+    };
+    // pass B: second digit inside every top bucket (k0 -> keys)
+    int rc = seg_pass(h.tb, ws->rsum(), ws->rpre(), segtot, 8 * (P - 2), k0, keys, false);
+    if (rc) return rc;
+    hipLaunchKernelGGL(hyb_buckets_kernel<0>, dim3(256), dim3(256), 0, s, h);
+    XSS_HIP_CHECK(hipGetLastError());
+    if (nan_last)
+        hipLaunchKernelGGL((hyb_local_kernel<KD, true>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart, h.bsize,
+                           keys, flip, uint32_t(nan_last));
+    else
+        hipLaunchKernelGGL((hyb_local_kernel<KD, false>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart, h.bsize,
+                           keys, flip, 0u);
+    XSS_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(hyb_large_ranges_kernel<0>, dim3(1), dim3(1024), 0, s, h);
+    XSS_HIP_CHECK(hipGetLastError());
+    // low digits of the large buckets (keys -> k0 -> keys; P - 2 is even)
+    const void *src = keys;
+    void *dst = k0;
+    for (int q = 0; q < P - 2; ++q) {
+        rc = seg_pass(h.td, rsum_d, rpre_d, segtot_d, 8 * q, src, dst, q == P - 3);
+        if (rc) return rc;
+        src = dst;
+        dst = (dst == k0) ? keys : static_cast<void *>(k0);
+    }
+    return XSS_OK;
+}
+
+}  // namespace xss
+
+namespace xss {
+
+// arrays of at least this many keys take the MSD-first path (xss_gpu_set_tuning)
+std::atomic<uint64_t> &hybrid_min_n();
+
+// x86simdsort::qsort on the device: MSD-first for large arrays of 4- and
+// 8-byte keys, LSD otherwise
+template <class KD>
+int run_qsort(void *keys, size_t n, bool hasnan, bool desc, hipStream_t s)
+{
+    if constexpr (sizeof(typename KD::U) >= 4) {
+        if (keys && n >= std::max<uint64_t>(hybrid_min_n().load(), 2) && n <= MAX_N) {
+            Workspace *ws = nullptr;
+            std::unique_lock<std::mutex> lock;
+            int rc = acquire_workspace(s, HybPlan<KD>::scratch_bytes(n), &ws, &lock);
+            if (rc) return rc;
+            return enqueue_hybrid<KD>(ws, ws->data, keys, n, hasnan, desc, s);
+        }
+    }
+    return run_sort<KD, PL_NONE>(keys, keys, nullptr, nullptr, nullptr, n, hasnan, desc, true, s);
+}
+
+}  // namespace xss

@@ -97,12 +97,14 @@ struct UpArgs {
     uint32_t ranges;
     const uint64_t *range_begin;  // segmented form: explicit key range of
     const uint64_t *range_end;    // each workgroup (null: blockIdx.x * range_keys)
+    const uint32_t *nr_dev;       // device-built range tables: the range count
 };
 
 template <class KD>
 __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
 {
     using U = typename KD::U;
+    if (a.nr_dev && blockIdx.x >= *a.nr_dev) return;  // surplus workgroup of a device-built table
     __shared__ uint32_t h[256];
     for (int i = threadIdx.x; i < 256; i += UP_BLOCK) h[i] = 0;
     __syncthreads();
@@ -459,6 +461,117 @@ __global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL) void downsweep
     for (; tile < tile1; ++tile)
         downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, uint64_t(tile) * TILE, a.n - tile * TILE, 0, run);
 }
+
+// ---------------------------------------------------------------- segmented passes
+//
+// A segmented LSD pass runs over a table of ranges: every range lies inside
+// one segment, and a segment's ranges are consecutive in the table.  Used by
+// the batch form (host-built tables) and by the MSD-first one-array sort
+// (tables built on the device; their count is then read from nr_dev and
+// grids are launched at the table's capacity, surplus workgroups exiting).
+
+struct SegScanArgs {
+    const uint32_t *rsum;    // [256][stride]
+    uint32_t *rpre;          // [256][stride] exclusive prefix within the segment
+    uint32_t *segtot;        // [segments][256] digit totals per segment
+    const uint32_t *head;    // [nr] 1 at the first range of a segment
+    const uint32_t *rbseg;   // [nr] segment index of each range
+    uint32_t nr;             // ranges (when nr_dev is null)
+    uint32_t stride;         // row length of rsum / rpre
+    const uint32_t *nr_dev;  // device-built tables: the range count
+};
+
+// workgroup d: segmented exclusive scan of column d over the ranges
+template <int = 0>
+__global__ __launch_bounds__(256) void seg_scan_kernel(SegScanArgs a)
+{
+    __shared__ uint32_t s_sum[256];
+    __shared__ uint32_t s_flag[256];
+    const uint32_t d = blockIdx.x, t = threadIdx.x;
+    const uint32_t nr = a.nr_dev ? *a.nr_dev : a.nr;
+    const uint32_t per = (nr + 255) / 256;
+    const uint32_t c0 = min(nr, t * per), c1 = min(nr, c0 + per);
+    const uint32_t *col = a.rsum + size_t(d) * a.stride;
+    // chunk summary: sum since the chunk's last head, and whether it has one
+    uint32_t sum = 0, flag = 0;
+    for (uint32_t r = c0; r < c1; ++r) {
+        if (a.head[r]) {
+            sum = 0;
+            flag = 1;
+        }
+        sum += col[r];
+    }
+    s_sum[t] = sum;
+    s_flag[t] = flag;
+    __syncthreads();
+    // inclusive segmented scan over the 256 chunk summaries (Hillis-Steele)
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        uint32_t ps = 0, pf = 0;
+        if (t >= off) {
+            ps = s_sum[t - off];
+            pf = s_flag[t - off];
+        }
+        __syncthreads();
+        if (t >= off && !flag) {
+            sum += ps;
+            flag |= pf;
+        }
+        s_sum[t] = sum;
+        s_flag[t] = flag;
+        __syncthreads();
+    }
+    uint32_t carry = t > 0 ? s_sum[t - 1] : 0u;  // running sum entering this chunk
+    uint32_t *out = a.rpre + size_t(d) * a.stride;
+    for (uint32_t r = c0; r < c1; ++r) {
+        if (a.head[r]) carry = 0;
+        out[r] = carry;
+        carry += col[r];
+        if (r + 1 == nr || a.head[r + 1]) a.segtot[size_t(a.rbseg[r]) * 256 + d] = carry;
+    }
+}
+
+struct SegRanges {
+    const uint64_t *begin;   // [nr] first key of each range
+    const uint64_t *end;     // [nr]
+    const uint64_t *sbase;   // [nr] first key of the range's segment
+    const uint32_t *rbseg;   // [nr] segment index
+    const uint32_t *rpre;    // [256][stride]
+    const uint32_t *segtot;  // [segments][256]
+    uint32_t nr;
+    uint32_t stride;
+    const uint32_t *nr_dev;  // device-built tables: the range count
+};
+
+// Tile shape of the segmented passes: 512 threads x 16 items (8 for items
+// wider than 8 bytes), smaller than the one-array pass for 4-byte keys
+// because a segment of a few tiles wastes less of a smaller last tile.  The
+// host-side plan (shape_of below) cuts ranges with the same tile size.
+// Keys-only 4-byte tiles get a 128-VGPR budget (two workgroups per CU).
+template <class KD, int PL>
+struct SegShape {
+    static constexpr int BYTES = ScatterClass<KD, PL>::bytes;
+    static constexpr int BLOCK = 512;
+    static constexpr int ITEMS = BYTES > 8 ? 8 : 16;
+    static constexpr int WAVES = BYTES == 4 ? 4 : 1;
+};
+
+template <class KD, int PL, int BLOCK, int ITEMS, bool NL>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SegShape<KD, PL>::WAVES)))
+void seg_downsweep_kernel(PassArgs a, SegRanges r)
+{
+    constexpr int TILE = BLOCK * ITEMS;
+    __shared__ DownsweepSmem<KD, PL, BLOCK, ITEMS> sm;
+    const uint32_t tid = threadIdx.x, b = blockIdx.x;
+    if (r.nr_dev && b >= *r.nr_dev) return;
+    const uint64_t rb = r.begin[b], re = r.end[b], sbase = r.sbase[b];
+    // digit starts inside this range's segment
+    const uint32_t dstart = digit_starts(r.segtot + size_t(r.rbseg[b]) * 256, sm.wsum);
+    uint64_t run = tid < 256 ? sbase + dstart + r.rpre[size_t(tid) * r.stride + b] : 0u;
+    uint64_t t = rb;
+    for (; t + TILE <= re; t += TILE) downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(a, sm, t, TILE, sbase, run);
+    if (t < re) downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, t, uint32_t(re - t), sbase, run);
+}
+
 
 // ---------------------------------------------------------------- block sort
 

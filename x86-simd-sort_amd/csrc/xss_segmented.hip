@@ -24,101 +24,6 @@ using namespace xss;
 
 namespace {
 
-struct SegScanArgs {
-    const uint32_t *rsum;    // [256][nr]
-    uint32_t *rpre;          // [256][nr] exclusive prefix within the segment
-    uint32_t *segtot;        // [nbig][256] digit totals per large segment
-    const uint32_t *head;    // [nr] 1 at the first range of a segment
-    const uint32_t *rbseg;   // [nr] large-segment index of each range
-    uint32_t nr;
-};
-
-// workgroup d: segmented exclusive scan of column d over the ranges
-__global__ __launch_bounds__(256) void seg_scan_kernel(SegScanArgs a)
-{
-    __shared__ uint32_t s_sum[256];
-    __shared__ uint32_t s_flag[256];
-    const uint32_t d = blockIdx.x, t = threadIdx.x;
-    const uint32_t per = (a.nr + 255) / 256;
-    const uint32_t c0 = min(a.nr, t * per), c1 = min(a.nr, c0 + per);
-    const uint32_t *col = a.rsum + size_t(d) * a.nr;
-    // chunk summary: sum since the chunk's last head, and whether it has one
-    uint32_t sum = 0, flag = 0;
-    for (uint32_t r = c0; r < c1; ++r) {
-        if (a.head[r]) {
-            sum = 0;
-            flag = 1;
-        }
-        sum += col[r];
-    }
-    s_sum[t] = sum;
-    s_flag[t] = flag;
-    __syncthreads();
-    // inclusive segmented scan over the 256 chunk summaries (Hillis-Steele)
-    for (uint32_t off = 1; off < 256; off <<= 1) {
-        uint32_t ps = 0, pf = 0;
-        if (t >= off) {
-            ps = s_sum[t - off];
-            pf = s_flag[t - off];
-        }
-        __syncthreads();
-        if (t >= off && !flag) {
-            sum += ps;
-            flag |= pf;
-        }
-        s_sum[t] = sum;
-        s_flag[t] = flag;
-        __syncthreads();
-    }
-    uint32_t carry = t > 0 ? s_sum[t - 1] : 0u;  // running sum entering this chunk
-    uint32_t *out = a.rpre + size_t(d) * a.nr;
-    for (uint32_t r = c0; r < c1; ++r) {
-        if (a.head[r]) carry = 0;
-        out[r] = carry;
-        carry += col[r];
-        if (r + 1 == a.nr || a.head[r + 1]) a.segtot[size_t(a.rbseg[r]) * 256 + d] = carry;
-    }
-}
-
-struct SegRanges {
-    const uint64_t *begin;   // [nr] first key of each range
-    const uint64_t *end;     // [nr]
-    const uint64_t *sbase;   // [nr] first key of the range's segment
-    const uint32_t *rbseg;   // [nr] large-segment index
-    const uint32_t *rpre;    // [256][nr]
-    const uint32_t *segtot;  // [nbig][256]
-    uint32_t nr;
-};
-
-// Tile shape of the segmented passes: 512 threads x 16 items (8 for items
-// wider than 8 bytes), smaller than the one-array pass for 4-byte keys
-// because a segment of a few tiles wastes less of a smaller last tile.  The
-// host-side plan (shape_of below) cuts ranges with the same tile size.
-// Keys-only 4-byte tiles get a 128-VGPR budget (two workgroups per CU).
-template <class KD, int PL>
-struct SegShape {
-    static constexpr int BYTES = ScatterClass<KD, PL>::bytes;
-    static constexpr int BLOCK = 512;
-    static constexpr int ITEMS = BYTES > 8 ? 8 : 16;
-    static constexpr int WAVES = BYTES == 4 ? 4 : 1;
-};
-
-template <class KD, int PL, int BLOCK, int ITEMS, bool NL>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SegShape<KD, PL>::WAVES)))
-void seg_downsweep_kernel(PassArgs a, SegRanges r)
-{
-    constexpr int TILE = BLOCK * ITEMS;
-    __shared__ DownsweepSmem<KD, PL, BLOCK, ITEMS> sm;
-    const uint32_t tid = threadIdx.x, b = blockIdx.x;
-    const uint64_t rb = r.begin[b], re = r.end[b], sbase = r.sbase[b];
-    // digit starts inside this range's segment
-    const uint32_t dstart = digit_starts(r.segtot + size_t(r.rbseg[b]) * 256, sm.wsum);
-    uint64_t run = tid < 256 ? sbase + dstart + r.rpre[size_t(tid) * r.nr + b] : 0u;
-    uint64_t t = rb;
-    for (; t + TILE <= re; t += TILE) downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(a, sm, t, TILE, sbase, run);
-    if (t < re) downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, t, uint32_t(re - t), sbase, run);
-}
-
 struct Tuning {
     uint32_t small_max;    // segments up to this length: LDS bitonic
     uint32_t range_tiles;  // tiles per workgroup range in larger segments
@@ -189,10 +94,10 @@ int seg_run(xss_segplan *p, void *keys, uint32_t *idx32, bool hasnan, bool desc,
             hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(p->nr), dim3(UP_BLOCK), 0, s, u);
         }
         XSS_HIP_CHECK(hipGetLastError());
-        SegScanArgs sc{p->rsum, p->rpre, p->segtot, p->r_head, p->r_bseg, p->nr};
+        SegScanArgs sc{p->rsum, p->rpre, p->segtot, p->r_head, p->r_bseg, p->nr, p->nr, nullptr};
         {
             KernelTimer t("seg_scan", s);
-            hipLaunchKernelGGL(seg_scan_kernel, dim3(256), dim3(256), 0, s, sc);
+            hipLaunchKernelGGL(seg_scan_kernel<0>, dim3(256), dim3(256), 0, s, sc);
         }
         XSS_HIP_CHECK(hipGetLastError());
         PassArgs a{};
@@ -208,7 +113,7 @@ int seg_run(xss_segplan *p, void *keys, uint32_t *idx32, bool hasnan, bool desc,
         } else {
             a.keys_out = (pass % 2 == 0) ? static_cast<void *>(k0) : keys;
         }
-        SegRanges r{p->r_begin, p->r_end, p->r_sbase, p->r_bseg, p->rpre, p->segtot, p->nr};
+        SegRanges r{p->r_begin, p->r_end, p->r_sbase, p->r_bseg, p->rpre, p->segtot, p->nr, p->nr, nullptr};
         {
             KernelTimer t("seg_downsweep", s);
             if (nan_last)

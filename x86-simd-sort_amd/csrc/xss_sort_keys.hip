@@ -1,14 +1,16 @@
 // xss_sort_keys.hip -- C-ABI entry points xss_gpu_qsort_<T> and
-// xss_gpu_argsort_<T> (x86simdsort::qsort / argsort, lib/x86simdsort.h:15-18,36-39).
-#include "xss_sort.hpp"
+// xss_gpu_argsort_<T> (x86simdsort::qsort / argsort, lib/x86simdsort.h:15-18,36-39),
+// and the library's tuning switches.
+#include <string>
+
+#include "xss_hybrid.hpp"
 
 using namespace xss;
 
 #define XSS_DEF_SORT(tag, T, CT)                                                                      \
     extern "C" int xss_gpu_qsort_##tag(T *arr, size_t n, int hasnan, int descending, xss_stream_t s)  \
     {                                                                                                 \
-        return run_sort<TypeKey<CT>::KD, PL_NONE>(arr, arr, nullptr, nullptr, nullptr, n, hasnan != 0, \
-                                                   descending != 0, true, s);                         \
+        return run_qsort<TypeKey<CT>::KD>(arr, n, hasnan != 0, descending != 0, s);                   \
     }                                                                                                 \
     extern "C" int xss_gpu_argsort_##tag(const T *arr, uint64_t *arg, size_t n, int hasnan,           \
                                          int descending, xss_stream_t s)                              \
@@ -26,3 +28,21 @@ XSS_DEF_SORT(f32, float, float)
 XSS_DEF_SORT(u64, uint64_t, uint64_t)
 XSS_DEF_SORT(i64, int64_t, int64_t)
 XSS_DEF_SORT(f64, double, double)
+
+namespace xss {
+std::atomic<uint64_t> &hybrid_min_n()
+{
+    static std::atomic<uint64_t> v{HYB_MIN_N};
+    return v;
+}
+}  // namespace xss
+
+extern "C" int xss_gpu_set_tuning(const char *name, uint64_t value)
+{
+    if (!name) return set_error(XSS_ERR_INVALID, "null tuning name");
+    if (std::string(name) == "hybrid_min_n") {
+        hybrid_min_n().store(value);
+        return XSS_OK;
+    }
+    return set_error(XSS_ERR_INVALID, "unknown tuning '%s'", name);
+}

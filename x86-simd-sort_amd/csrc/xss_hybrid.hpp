@@ -184,7 +184,6 @@ __global__ __launch_bounds__(HYB_BLOCK) void hyb_local_kernel(const uint32_t *bs
     using U = typename KD::U;
     constexpr int P = KD::PASSES;
     constexpr int WAVES = HYB_BLOCK / 64;
-    constexpr int WT = 64 * HYB_ITEMS;
     __shared__ U s_keys[HYB_CAP];
     __shared__ uint32_t s_off[WAVES * 256];
     __shared__ uint32_t s_wsum[WAVES];
@@ -203,32 +202,40 @@ __global__ __launch_bounds__(HYB_BLOCK) void hyb_local_kernel(const uint32_t *bs
         return;
     }
     const U flip = U(flip64);
-    const uint32_t lbase = wave * WT + lane;
+    // the bucket is spread over all waves: `it` items per thread (uniform),
+    // wave w holding keys [w * 64 * it, (w + 1) * 64 * it)
+    const uint32_t it = (m + HYB_BLOCK - 1) / HYB_BLOCK;
+    const uint32_t lbase = wave * 64 * it + lane;
     U key[HYB_ITEMS];
 #pragma unroll
     for (int i = 0; i < HYB_ITEMS; ++i) {
         const uint32_t idx = lbase + i * 64;
-        key[i] = idx < m ? keys[idx] : U(0);
+        key[i] = (uint32_t(i) < it && idx < m) ? keys[idx] : U(0);
     }
 #pragma unroll 1
     for (int p = 0; p < P - 2; ++p) {
         const int shift = 8 * p;
         for (uint32_t i = tid; i < WAVES * 256; i += HYB_BLOCK) s_off[i] = 0;
         __syncthreads();  // counters cleared; the previous pass's reads of s_keys are done
-        // rank: stable within the bucket (wave-striped order), per-wave counters;
-        // padding slots take digit 255 and come after every key
+        // rank: stable within the bucket (wave-striped order), per-wave
+        // counters; the lanes holding one digit reserve their slots with one
+        // LDS atomic by the lowest of them, so successive items do not wait
+        // on each other's counter update.  Padding takes digit 255 and comes
+        // after every key.
         uint32_t rd[HYB_ITEMS];
         uint32_t *wc = s_off + wave * 256;
 #pragma unroll
         for (int i = 0; i < HYB_ITEMS; ++i) {
-            const uint32_t idx = lbase + i * 64;
-            const uint32_t d = idx < m ? KD::digit(key[i], flip, NL, shift) : 255u;
-            const uint64_t peers = match_any8(d);
-            const uint32_t below = lanes_below(peers);
-            const uint32_t tot = uint32_t(__popcll(peers));
-            const uint32_t c = wc[d];
-            rd[i] = (c + below) | (d << 16);
-            if (below + 1 == tot) wc[d] = c + tot;
+            if (uint32_t(i) < it) {
+                const uint32_t idx = lbase + i * 64;
+                const uint32_t d = idx < m ? KD::digit(key[i], flip, NL, shift) : 255u;
+                const uint64_t peers = match_any8(d);
+                const uint32_t below = lanes_below(peers);
+                uint32_t old = 0;
+                if (below == 0) old = atomicAdd(&wc[d], uint32_t(__popcll(peers)));
+                old = __shfl(old, int(__builtin_ctzll(peers)));
+                rd[i] = (old + below) | (d << 16);
+            }
         }
         __syncthreads();
         // digit counts, their exclusive scan, and the slot base of every (wave, digit)
@@ -258,17 +265,20 @@ __global__ __launch_bounds__(HYB_BLOCK) void hyb_local_kernel(const uint32_t *bs
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < HYB_ITEMS; ++i) {
-            const uint32_t d = rd[i] >> 16;
-            s_keys[wc[d] + (rd[i] & 0xffffu)] = key[i];
+            if (uint32_t(i) < it) {
+                const uint32_t d = rd[i] >> 16;
+                s_keys[wc[d] + (rd[i] & 0xffffu)] = key[i];
+            }
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < HYB_ITEMS; ++i) key[i] = s_keys[lbase + i * 64];
+        for (int i = 0; i < HYB_ITEMS; ++i)
+            if (uint32_t(i) < it) key[i] = s_keys[lbase + i * 64];
     }
 #pragma unroll
     for (int i = 0; i < HYB_ITEMS; ++i) {
         const uint32_t idx = lbase + i * 64;
-        if (idx < m) keys[idx] = finish(key[i]);
+        if (uint32_t(i) < it && idx < m) keys[idx] = finish(key[i]);
     }
 }
 

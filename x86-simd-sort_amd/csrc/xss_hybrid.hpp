@@ -28,8 +28,19 @@
 
 namespace xss {
 
-constexpr int HYB_BLOCK = 512;
-constexpr int HYB_ITEMS = 12;
+#ifndef XSS_HYB_BLOCK
+#define XSS_HYB_BLOCK 512
+#endif
+#ifndef XSS_HYB_ITEMS
+#define XSS_HYB_ITEMS 12
+#endif
+#ifndef XSS_HYB_WAVES
+#define XSS_HYB_WAVES 8
+#endif
+// waves per SIMD the local sort is compiled for (its SGPRs alone would cap it at 7)
+#define XSS_HYB_OCC __attribute__((amdgpu_waves_per_eu(XSS_HYB_WAVES)))
+constexpr int HYB_BLOCK = XSS_HYB_BLOCK;  // local sort workgroup (build-time knobs)
+constexpr int HYB_ITEMS = XSS_HYB_ITEMS;
 constexpr uint32_t HYB_CAP = HYB_BLOCK * HYB_ITEMS;  // largest bucket sorted in LDS
 constexpr uint32_t HYB_BUCKETS = 65536;              // buckets of the top two digits
 constexpr size_t HYB_MIN_N = size_t(1) << 25;        // smaller arrays take the LSD path
@@ -178,7 +189,7 @@ __global__ __launch_bounds__(1024) void hyb_large_ranges_kernel(HybArgs a)
 // one workgroup per bucket of at most HYB_CAP keys: LSD radix by the low
 // P - 2 digits in LDS (the top two are the bucket's), written back in place
 template <class KD, bool NL>
-__global__ __launch_bounds__(HYB_BLOCK) void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize,
+__global__ __launch_bounds__(HYB_BLOCK) XSS_HYB_OCC void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize,
                                                               void *keys_v, uint64_t flip64, uint32_t canon)
 {
     using U = typename KD::U;
@@ -285,6 +296,8 @@ __global__ __launch_bounds__(HYB_BLOCK) void hyb_local_kernel(const uint32_t *bs
 template <class KD>
 struct HybPlan {
     using U = typename KD::U;
+    // tile shape of the segmented passes (8K-key tiles for 4-byte keys
+    // measured faster here than the one-array pass's 16K)
     using SS = SegShape<KD, PL_NONE>;
     static constexpr uint64_t SEG_TILE = uint64_t(SS::BLOCK) * SS::ITEMS;
     // pass B's table must fit the workspace's [256][MAX_RANGES] range tables
@@ -312,8 +325,8 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasn
     using U = typename KD::U;
     using Plan = SortPlan<KD, PL_NONE>;
     using PS = PassShape<KD, PL_NONE>;
-    using SS = SegShape<KD, PL_NONE>;
     using HP = HybPlan<KD>;
+    using SS = typename HP::SS;
     constexpr int P = KD::PASSES;
     static_assert(P >= 4, "two top digits and at least two low digits");
     const uint64_t flip = desc ? ~0ull : 0ull;

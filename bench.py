@@ -11,19 +11,27 @@ the reference's benchmark (benchmarks/bench-qsort.hpp:12-16, PauseTiming)
 the restore is outside the timed sort: each step is bracketed by HIP events
 on the stream the library launches on, and the JSON reports their sum.
 
+The sort of 2^28 4-byte keys runs MSD-first: an LSD pass on the top digit,
+a segmented pass on the second digit inside every top bucket, then every
+bucket of the top 16 bits sorted by its low 16 bits in LDS (buckets too large
+for LDS take segmented passes).
+
 The JSON line carries:
   value/ms_per_step   whole-sort throughput (Gkeys/s) and time per sort
-  roofline            the dominant kernel (one LSD scatter pass): its
-                      algorithmic bytes per launch (N * 2 * key bytes: every
-                      key read once and written once) over its average launch
-                      time from the library's per-kernel HIP events, against
+  roofline            the dominant kernel: its algorithmic bytes per launch
+                      (N * 2 * key bytes for kernels that read and write
+                      every key once, N * key bytes for histogram kernels)
+                      over its average launch time from the library's
+                      per-kernel HIP events on the launch stream, against
                       MI355X's 8 TB/s; `traffic` is the measured HBM bytes per
                       launch from profiles/ when a rocprofv3 --pmc summary is
                       present there
   pipeline            the whole sort against the roofline at the algorithmic
-                      (1 + 2P) * key bytes per key of an 8-bit LSD sort; this
-                      design moves 3P * key bytes per key (each pass = one
-                      upsweep read + one downsweep read and write)
+                      (1 + 2P) * key bytes per key of an 8-bit LSD sort; the
+                      MSD-first design moves 8 * key bytes per key (two
+                      passes of histogram read + read + write, then one read
+                      and one write); per-kernel ms per sort; and the same
+                      sort timed on the LSD path (four passes) for reference
   segmented           the batch form: 8192 uneven float32 arrays (~775M keys)
                       sorted by one graph-captured plan against a loop of
                       plain per-array calls on the same GPU
@@ -196,8 +204,10 @@ def segmented_batch(torch, xss_amd, L, tuning, reps=3):
     return out
 
 
-def traffic_from_profiles(dtype, n):
-    p = os.path.join(REPO, "profiles", f"downsweep_{dtype}_256M_traffic.json")
+def traffic_from_profiles(kernel, dtype, n):
+    """Measured HBM bytes per launch of `kernel` from a rocprofv3 --pmc
+    summary in profiles/ (None when there is none for this size)."""
+    p = os.path.join(REPO, "profiles", f"{kernel}_{dtype}_256M_traffic.json")
     if not os.path.exists(p):
         return None
     try:
@@ -271,17 +281,48 @@ def main():
         v = out.to(torch.int64) & 0xFFFFFFFF
         ok = bool((v[1:] >= v[:-1]).all().item())
 
-    pass_ms, pass_cnt = kt.get("downsweep", (0.0, 0))
-    pass_avg_ms = pass_ms / max(1, pass_cnt)
-    pass_bytes = n * 2 * key_bytes
-    achieved = pass_bytes / (pass_avg_ms / 1e3) / 1e9 if pass_avg_ms > 0 else 0.0
+    # The dominant kernel of the sort, priced at its algorithmic bytes per
+    # launch: scatter-type kernels read and write every key once, histogram
+    # kernels read every key once (the hybrid's per-bucket LDS sort also
+    # reads and writes each key once).
+    alg_per_launch = {"downsweep": 2 * n * key_bytes, "hyb_msd_downsweep": 2 * n * key_bytes,
+                      "hyb_local": 2 * n * key_bytes, "upsweep": n * key_bytes, "hyb_msd_upsweep": n * key_bytes}
+    dom = max((k for k in alg_per_launch if k in kt), key=lambda k: kt[k][0])
+    dom_ms, dom_cnt = kt[dom]
+    dom_avg_ms = dom_ms / max(1, dom_cnt)
+    dom_bytes = alg_per_launch[dom]
+    achieved = dom_bytes / (dom_avg_ms / 1e3) / 1e9 if dom_avg_ms > 0 else 0.0
     P = 4
+    hybrid = "hyb_local" in kt
     # algorithmic minimum of an 8-bit LSD sort: one histogram read + P x (read + write)
     pipe_bytes = n * (1 + 2 * P) * key_bytes
-    # what this design moves: P x (upsweep read + downsweep read + write)
-    moved_bytes = n * 3 * P * key_bytes
+    # what this design moves: MSD-first = 2 x (histogram read + read + write)
+    # + the LDS bucket sort's read + write (buckets too large for LDS add
+    # segmented passes); LSD = P x (histogram read + read + write)
+    moved_bytes = n * (8 if hybrid else 3 * P) * key_bytes
     pipe_gbs = pipe_bytes / (ms_per_step / 1e3) / 1e9
     gkeys = n / (ms_per_step / 1e3) / 1e9
+    kernel_desc = {"downsweep": "downsweep_kernel (LSD scatter pass, top digit)",
+                   "hyb_msd_downsweep": "seg_downsweep_kernel (second-digit pass inside top buckets)",
+                   "hyb_local": "hyb_local_kernel (LDS radix of each 16-bit bucket, low 16 bits)",
+                   "upsweep": "upsweep_kernel (digit histograms)",
+                   "hyb_msd_upsweep": "upsweep_kernel (second-digit histograms)"}[dom]
+    # the same sort on the LSD path (four 8-bit passes), for reference
+    lsd_ms = None
+    if hybrid:
+        L.xss_gpu_set_tuning.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
+        L.xss_gpu_set_tuning(b"hybrid_min_n", 1 << 62)
+        ts = []
+        for _ in range(4):
+            restore()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            xss_amd.qsort(arr)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        L.xss_gpu_set_tuning(b"hybrid_min_n", 1 << 25)
+        lsd_ms = round(statistics.median(ts[1:]), 4)
     line = {
         "metric": METRIC,
         "value": round(gkeys, 3),
@@ -297,15 +338,16 @@ def main():
         "data": "synthetic (uniform full-range bits, finite, seeded on device)",
         "config": {"workload": f"qsort<{'float' if dt == 'f32' else 'uint32_t'}> of 2^28 keys in HBM, "
                                f"ascending, hasnan=false", "n": n, "key_bytes": key_bytes,
-                   "digit_bits": 8, "passes": P, "restore_excluded": True},
+                   "digit_bits": 8, "design": "msd_hybrid" if hybrid else "lsd", "restore_excluded": True},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dt, n),
-                     "kernel": "downsweep_kernel (one LSD scatter pass)", "avg_launch_ms": round(pass_avg_ms, 4),
-                     "bytes_per_launch_alg": pass_bytes},
-        "pipeline": {"bytes_per_sort_alg": pipe_bytes, "bytes_per_sort_moved": moved_bytes,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom, dt, n),
+                     "kernel": kernel_desc, "avg_launch_ms": round(dom_avg_ms, 4),
+                     "bytes_per_launch_alg": dom_bytes},
+        "pipeline": {"bytes_per_sort_alg_lsd8": pipe_bytes, "bytes_per_sort_moved": moved_bytes,
                      "achieved_GBs": round(pipe_gbs, 1),
                      "frac": round(pipe_gbs / HBM_PEAK_GBS, 4),
-                     "kernels_ms_per_sort": {k: round(v[0] / args.steps, 4) for k, v in kt.items()}},
+                     "kernels_ms_per_sort": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+                     "lsd_path_ms_per_sort": lsd_ms},
         "wall_s_timed_loop_incl_restore": round(wall, 4),
         "sorted_ok": ok,
     }

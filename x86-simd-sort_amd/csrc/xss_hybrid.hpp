@@ -371,22 +371,28 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasn
     h.rk_b = HP::rk_b(n);
     h.rk_d = HP::rk_d(n);
 
-    KernelTimer timer("hybrid_sort", s);
     // pass A: top digit over the whole array (keys -> k0)
     {
         const uint32_t rtiles = Plan::range_tiles(n), ranges = Plan::ranges(n);
         UpArgs u{};
         u.keys = keys; u.n = nn; u.range_keys = rtiles * PS::TILE; u.shift = 8 * (P - 1);
         u.nan_last = nan_last; u.flip = flip; u.rsum = ws->rsum(); u.ranges = ranges;
-        hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
+        {
+            KernelTimer t("upsweep", s);
+            hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
+        }
         XSS_HIP_CHECK(hipGetLastError());
         ScanArgs sc{};
         sc.rsum = ws->rsum(); sc.rpre = ws->rpre(); sc.total = ws->total(); sc.ranges = ranges;
-        hipLaunchKernelGGL(range_scan_kernel<0>, dim3(256), dim3(SCAN_BLOCK), 0, s, sc);
+        {
+            KernelTimer t("range_scan", s);
+            hipLaunchKernelGGL(range_scan_kernel<0>, dim3(256), dim3(SCAN_BLOCK), 0, s, sc);
+        }
         XSS_HIP_CHECK(hipGetLastError());
         PassArgs a{};
         a.keys_in = keys; a.keys_out = k0; a.n = nn; a.shift = 8 * (P - 1); a.flip = flip;
         a.nan_last = nan_last; a.range_tiles = rtiles; a.ranges = ranges; a.rpre = ws->rpre(); a.total = ws->total();
+        KernelTimer t("downsweep", s);
         if (nan_last)
             hipLaunchKernelGGL((downsweep_kernel<KD, PL_NONE, PS::BLOCK, PS::ITEMS, true>), dim3(ranges),
                                dim3(PS::BLOCK), 0, s, a);
@@ -395,51 +401,75 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasn
                                dim3(PS::BLOCK), 0, s, a);
         XSS_HIP_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(hyb_top_ranges_kernel<0>, dim3(1), dim3(256), 0, s, h);
+    {
+        KernelTimer t("hyb_tables", s);
+        hipLaunchKernelGGL(hyb_top_ranges_kernel<0>, dim3(1), dim3(256), 0, s, h);
+    }
     XSS_HIP_CHECK(hipGetLastError());
     // one segmented pass over a device-built table (src -> dst)
-    auto seg_pass = [&](const HybTable &t, uint32_t *rsum, uint32_t *rpre, uint32_t *stot, int shift, const void *src,
-                        void *dst, bool canon) -> int {
+    auto seg_pass = [&](const char *const names[3], const HybTable &t, uint32_t *rsum, uint32_t *rpre,
+                        uint32_t *stot, int shift, const void *src, void *dst, bool canon) -> int {
+        // the kernels walk the table's ranges with a bounded grid: a table
+        // that turns out (nearly) empty costs few workgroups
+        const uint32_t grid = std::min<uint32_t>(t.cap, MAX_RANGES);
         UpArgs u{};
         u.keys = src; u.n = nn; u.shift = uint32_t(shift); u.nan_last = nan_last; u.flip = flip;
         u.rsum = rsum; u.ranges = t.cap; u.range_begin = t.begin; u.range_end = t.end; u.nr_dev = t.nr;
-        hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(t.cap), dim3(UP_BLOCK), 0, s, u);
+        {
+            KernelTimer kt(names[0], s);
+            hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(grid), dim3(UP_BLOCK), 0, s, u);
+        }
         XSS_HIP_CHECK(hipGetLastError());
         SegScanArgs sc{rsum, rpre, stot, t.head, t.bseg, 0, t.cap, t.nr};
-        hipLaunchKernelGGL(seg_scan_kernel<0>, dim3(256), dim3(256), 0, s, sc);
+        {
+            KernelTimer kt(names[1], s);
+            hipLaunchKernelGGL(seg_scan_kernel<0>, dim3(256), dim3(256), 0, s, sc);
+        }
         XSS_HIP_CHECK(hipGetLastError());
         PassArgs a{};
         a.keys_in = src; a.keys_out = dst; a.n = nn; a.shift = uint32_t(shift); a.flip = flip;
         a.nan_last = nan_last; a.canon_nan = canon && nan_last;
         SegRanges r{t.begin, t.end, t.sbase, t.bseg, rpre, stot, 0, t.cap, t.nr};
+        KernelTimer kt(names[2], s);
         if (nan_last)
-            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL_NONE, SS::BLOCK, SS::ITEMS, true>), dim3(t.cap),
+            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL_NONE, SS::BLOCK, SS::ITEMS, true>), dim3(grid),
                                dim3(SS::BLOCK), 0, s, a, r);
         else
-            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL_NONE, SS::BLOCK, SS::ITEMS, false>), dim3(t.cap),
+            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL_NONE, SS::BLOCK, SS::ITEMS, false>), dim3(grid),
                                dim3(SS::BLOCK), 0, s, a, r);
         XSS_HIP_CHECK(hipGetLastError());
         return XSS_OK;
     };
     // pass B: second digit inside every top bucket (k0 -> keys)
-    int rc = seg_pass(h.tb, ws->rsum(), ws->rpre(), segtot, 8 * (P - 2), k0, keys, false);
+    static const char *const msd_names[3] = {"hyb_msd_upsweep", "hyb_msd_scan", "hyb_msd_downsweep"};
+    static const char *const low_names[3] = {"hyb_large_upsweep", "hyb_large_scan", "hyb_large_downsweep"};
+    int rc = seg_pass(msd_names, h.tb, ws->rsum(), ws->rpre(), segtot, 8 * (P - 2), k0, keys, false);
     if (rc) return rc;
-    hipLaunchKernelGGL(hyb_buckets_kernel<0>, dim3(256), dim3(256), 0, s, h);
+    {
+        KernelTimer t("hyb_tables", s);
+        hipLaunchKernelGGL(hyb_buckets_kernel<0>, dim3(256), dim3(256), 0, s, h);
+    }
     XSS_HIP_CHECK(hipGetLastError());
-    if (nan_last)
-        hipLaunchKernelGGL((hyb_local_kernel<KD, true>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart, h.bsize,
-                           keys, flip, uint32_t(nan_last));
-    else
-        hipLaunchKernelGGL((hyb_local_kernel<KD, false>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart, h.bsize,
-                           keys, flip, 0u);
+    {
+        KernelTimer t("hyb_local", s);
+        if (nan_last)
+            hipLaunchKernelGGL((hyb_local_kernel<KD, true>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart,
+                               h.bsize, keys, flip, uint32_t(nan_last));
+        else
+            hipLaunchKernelGGL((hyb_local_kernel<KD, false>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart,
+                               h.bsize, keys, flip, 0u);
+    }
     XSS_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(hyb_large_ranges_kernel<0>, dim3(1), dim3(1024), 0, s, h);
+    {
+        KernelTimer t("hyb_tables", s);
+        hipLaunchKernelGGL(hyb_large_ranges_kernel<0>, dim3(1), dim3(1024), 0, s, h);
+    }
     XSS_HIP_CHECK(hipGetLastError());
     // low digits of the large buckets (keys -> k0 -> keys; P - 2 is even)
     const void *src = keys;
     void *dst = k0;
     for (int q = 0; q < P - 2; ++q) {
-        rc = seg_pass(h.td, rsum_d, rpre_d, segtot_d, 8 * q, src, dst, q == P - 3);
+        rc = seg_pass(low_names, h.td, rsum_d, rpre_d, segtot_d, 8 * q, src, dst, q == P - 3);
         if (rc) return rc;
         src = dst;
         dst = (dst == k0) ? keys : static_cast<void *>(k0);

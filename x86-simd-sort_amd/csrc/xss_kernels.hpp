@@ -104,58 +104,63 @@ template <class KD>
 __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
 {
     using U = typename KD::U;
-    if (a.nr_dev && blockIdx.x >= *a.nr_dev) return;  // surplus workgroup of a device-built table
     __shared__ uint32_t h[256];
-    for (int i = threadIdx.x; i < 256; i += UP_BLOCK) h[i] = 0;
-    __syncthreads();
-    const U *keys = static_cast<const U *>(a.keys);
-    const U flip = U(a.flip);
-    const bool nl = a.nan_last != 0;
-    const int shift = int(a.shift);
-    const uint32_t r0 = a.range_begin ? uint32_t(a.range_begin[blockIdx.x]) : blockIdx.x * a.range_keys;
-    const uint32_t r1 = a.range_begin ? uint32_t(a.range_end[blockIdx.x]) : min(a.n, r0 + a.range_keys);
-    constexpr int VEC = 16 / sizeof(U);
-    // keys before the first 16-byte boundary are read one by one
-    const uint32_t mis = uint32_t(reinterpret_cast<uintptr_t>(keys + r0) & 15u);
-    const uint32_t head = min(r1 - r0, uint32_t(((16u - mis) & 15u) / sizeof(U)));
-    const uint32_t v0 = r0 + head;
-    const uint32_t nvec = (r1 - v0) / VEC;
-    const uint4 *kv = reinterpret_cast<const uint4 *>(keys + v0);
-    // count one 16-byte vector of keys; whole-wave runs of one digit (sorted
-    // or constant data) add once
-    auto count_vec = [&](const uint4 &q) {
-        const U *e = reinterpret_cast<const U *>(&q);
+    // one range per workgroup, or (device-built tables, launched with a
+    // bounded grid) every gridDim.x-th range
+    const uint32_t nr = a.nr_dev ? *a.nr_dev : gridDim.x;
+    for (uint32_t rg = blockIdx.x; rg < nr; rg += gridDim.x) {
+        __syncthreads();  // the previous range's histogram was written out
+        for (int i = threadIdx.x; i < 256; i += UP_BLOCK) h[i] = 0;
+        __syncthreads();
+        const U *keys = static_cast<const U *>(a.keys);
+        const U flip = U(a.flip);
+        const bool nl = a.nan_last != 0;
+        const int shift = int(a.shift);
+        const uint32_t r0 = a.range_begin ? uint32_t(a.range_begin[rg]) : rg * a.range_keys;
+        const uint32_t r1 = a.range_begin ? uint32_t(a.range_end[rg]) : min(a.n, r0 + a.range_keys);
+        constexpr int VEC = 16 / sizeof(U);
+        // keys before the first 16-byte boundary are read one by one
+        const uint32_t mis = uint32_t(reinterpret_cast<uintptr_t>(keys + r0) & 15u);
+        const uint32_t head = min(r1 - r0, uint32_t(((16u - mis) & 15u) / sizeof(U)));
+        const uint32_t v0 = r0 + head;
+        const uint32_t nvec = (r1 - v0) / VEC;
+        const uint4 *kv = reinterpret_cast<const uint4 *>(keys + v0);
+        // count one 16-byte vector of keys; whole-wave runs of one digit (sorted
+        // or constant data) add once
+        auto count_vec = [&](const uint4 &q) {
+            const U *e = reinterpret_cast<const U *>(&q);
 #pragma unroll
-        for (int j = 0; j < VEC; ++j) {
-            const uint32_t d = KD::digit(e[j], flip, nl, shift);
-            const uint64_t act = __ballot(1);
-            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-            if (__ballot(d != d0) == 0) {
-                if (lanes_below(act) == 0) atomicAdd(&h[d0], uint32_t(__popcll(act)));
-            } else {
-                atomicAdd(&h[d], 1u);
+            for (int j = 0; j < VEC; ++j) {
+                const uint32_t d = KD::digit(e[j], flip, nl, shift);
+                const uint64_t act = __ballot(1);
+                const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+                if (__ballot(d != d0) == 0) {
+                    if (lanes_below(act) == 0) atomicAdd(&h[d0], uint32_t(__popcll(act)));
+                } else {
+                    atomicAdd(&h[d], 1u);
+                }
             }
+        };
+        // four loads in flight per thread before any is consumed
+        constexpr uint32_t UNR = 4;
+        uint32_t v = threadIdx.x;
+        for (; v + (UNR - 1) * UP_BLOCK < nvec; v += UNR * UP_BLOCK) {
+            uint4 q[UNR];
+#pragma unroll
+            for (uint32_t u = 0; u < UNR; ++u) q[u] = kv[v + u * UP_BLOCK];
+#pragma unroll
+            for (uint32_t u = 0; u < UNR; ++u) count_vec(q[u]);
         }
-    };
-    // four loads in flight per thread before any is consumed
-    constexpr uint32_t UNR = 4;
-    uint32_t v = threadIdx.x;
-    for (; v + (UNR - 1) * UP_BLOCK < nvec; v += UNR * UP_BLOCK) {
-        uint4 q[UNR];
-#pragma unroll
-        for (uint32_t u = 0; u < UNR; ++u) q[u] = kv[v + u * UP_BLOCK];
-#pragma unroll
-        for (uint32_t u = 0; u < UNR; ++u) count_vec(q[u]);
+        for (; v < nvec; v += UP_BLOCK) count_vec(kv[v]);
+        const uint32_t t0 = v0 + nvec * VEC;
+        const uint32_t rest = head + (r1 - t0);
+        for (uint32_t r = threadIdx.x; r < rest; r += UP_BLOCK) {
+            const uint32_t i = r < head ? r0 + r : t0 + (r - head);
+            atomicAdd(&h[KD::digit(keys[i], flip, nl, shift)], 1u);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < 256; i += UP_BLOCK) a.rsum[size_t(i) * a.ranges + rg] = h[i];
     }
-    for (; v < nvec; v += UP_BLOCK) count_vec(kv[v]);
-    const uint32_t t0 = v0 + nvec * VEC;
-    const uint32_t rest = head + (r1 - t0);
-    for (uint32_t r = threadIdx.x; r < rest; r += UP_BLOCK) {
-        const uint32_t i = r < head ? r0 + r : t0 + (r - head);
-        atomicAdd(&h[KD::digit(keys[i], flip, nl, shift)], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < 256; i += UP_BLOCK) a.rsum[size_t(i) * a.ranges + blockIdx.x] = h[i];
 }
 
 // ---------------------------------------------------------------- range scan
@@ -561,15 +566,20 @@ void seg_downsweep_kernel(PassArgs a, SegRanges r)
 {
     constexpr int TILE = BLOCK * ITEMS;
     __shared__ DownsweepSmem<KD, PL, BLOCK, ITEMS> sm;
-    const uint32_t tid = threadIdx.x, b = blockIdx.x;
-    if (r.nr_dev && b >= *r.nr_dev) return;
-    const uint64_t rb = r.begin[b], re = r.end[b], sbase = r.sbase[b];
-    // digit starts inside this range's segment
-    const uint32_t dstart = digit_starts(r.segtot + size_t(r.rbseg[b]) * 256, sm.wsum);
-    uint64_t run = tid < 256 ? sbase + dstart + r.rpre[size_t(tid) * r.stride + b] : 0u;
-    uint64_t t = rb;
-    for (; t + TILE <= re; t += TILE) downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(a, sm, t, TILE, sbase, run);
-    if (t < re) downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, t, uint32_t(re - t), sbase, run);
+    const uint32_t tid = threadIdx.x;
+    // one range per workgroup, or (device-built tables, launched with a
+    // bounded grid) every gridDim.x-th range
+    const uint32_t nr = r.nr_dev ? *r.nr_dev : r.nr;
+    for (uint32_t b = blockIdx.x; b < nr; b += gridDim.x) {
+        __syncthreads();  // the previous range's LDS reads are done
+        const uint64_t rb = r.begin[b], re = r.end[b], sbase = r.sbase[b];
+        // digit starts inside this range's segment
+        const uint32_t dstart = digit_starts(r.segtot + size_t(r.rbseg[b]) * 256, sm.wsum);
+        uint64_t run = tid < 256 ? sbase + dstart + r.rpre[size_t(tid) * r.stride + b] : 0u;
+        uint64_t t = rb;
+        for (; t + TILE <= re; t += TILE) downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(a, sm, t, TILE, sbase, run);
+        if (t < re) downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, t, uint32_t(re - t), sbase, run);
+    }
 }
 
 

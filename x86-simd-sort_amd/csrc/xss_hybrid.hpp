@@ -323,8 +323,6 @@ template <class KD>
 int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasnan, bool desc, hipStream_t s)
 {
     using U = typename KD::U;
-    using Plan = SortPlan<KD, PL_NONE>;
-    using PS = PassShape<KD, PL_NONE>;
     using HP = HybPlan<KD>;
     using SS = typename HP::SS;
     constexpr int P = KD::PASSES;
@@ -373,33 +371,9 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasn
 
     // pass A: top digit over the whole array (keys -> k0)
     {
-        const uint32_t rtiles = Plan::range_tiles(n), ranges = Plan::ranges(n);
-        UpArgs u{};
-        u.keys = keys; u.n = nn; u.range_keys = rtiles * PS::TILE; u.shift = 8 * (P - 1);
-        u.nan_last = nan_last; u.flip = flip; u.rsum = ws->rsum(); u.ranges = ranges;
-        {
-            KernelTimer t("upsweep", s);
-            hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
-        }
-        XSS_HIP_CHECK(hipGetLastError());
-        ScanArgs sc{};
-        sc.rsum = ws->rsum(); sc.rpre = ws->rpre(); sc.total = ws->total(); sc.ranges = ranges;
-        {
-            KernelTimer t("range_scan", s);
-            hipLaunchKernelGGL(range_scan_kernel<0>, dim3(256), dim3(SCAN_BLOCK), 0, s, sc);
-        }
-        XSS_HIP_CHECK(hipGetLastError());
-        PassArgs a{};
-        a.keys_in = keys; a.keys_out = k0; a.n = nn; a.shift = 8 * (P - 1); a.flip = flip;
-        a.nan_last = nan_last; a.range_tiles = rtiles; a.ranges = ranges; a.rpre = ws->rpre(); a.total = ws->total();
-        KernelTimer t("downsweep", s);
-        if (nan_last)
-            hipLaunchKernelGGL((downsweep_kernel<KD, PL_NONE, PS::BLOCK, PS::ITEMS, true>), dim3(ranges),
-                               dim3(PS::BLOCK), 0, s, a);
-        else
-            hipLaunchKernelGGL((downsweep_kernel<KD, PL_NONE, PS::BLOCK, PS::ITEMS, false>), dim3(ranges),
-                               dim3(PS::BLOCK), 0, s, a);
-        XSS_HIP_CHECK(hipGetLastError());
+        const int rc = enqueue_lsd_pass<KD, PL_NONE>(ws, keys, nullptr, k0, nullptr, nullptr, n, 8 * (P - 1), flip,
+                                                     nan_last, true, false, false, s);
+        if (rc) return rc;
     }
     {
         KernelTimer t("hyb_tables", s);

@@ -437,16 +437,20 @@ struct ScatterClass {
         int(sizeof(typename KD::U)) + (PL == PL_NONE ? 0 : int(sizeof(typename PayloadT<PL>::type)));
     static constexpr int value = bytes <= 2 ? 0 : bytes <= 4 ? 1 : bytes <= 8 ? 2 : 3;
 };
-template <class KD, int PL>
+// 512-thread tiles of 4-byte keys (the shape for smaller arrays) get 128
+// VGPRs = two workgroups per CU.
+template <class KD, int PL, int BLOCK>
 struct ScatterWaves {
     static constexpr int c = ScatterClass<KD, PL>::value;
-    static constexpr int value = c == 0 ? XSS_SCATTER_WAVES_C0 : c == 1 ? XSS_SCATTER_WAVES_C1
+    static constexpr int value = (BLOCK <= 512 && ScatterClass<KD, PL>::bytes == 4) ? 4
+                               : c == 0 ? XSS_SCATTER_WAVES_C0 : c == 1 ? XSS_SCATTER_WAVES_C1
                                : c == 2 ? XSS_SCATTER_WAVES_C2 : XSS_SCATTER_WAVES_C3;
 };
-#define XSS_SCATTER_OCCUPANCY(KD, PL) __attribute__((amdgpu_waves_per_eu(ScatterWaves<KD, PL>::value)))
+#define XSS_SCATTER_OCCUPANCY(KD, PL, BLOCK) \
+    __attribute__((amdgpu_waves_per_eu(ScatterWaves<KD, PL, BLOCK>::value)))
 
 template <class KD, int PL, int BLOCK, int ITEMS, bool NL>
-__global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL) void downsweep_kernel(PassArgs a)
+__global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL, BLOCK) void downsweep_kernel(PassArgs a)
 {
     constexpr int TILE = BLOCK * ITEMS;
     static_assert(BLOCK >= 256 && BLOCK % 64 == 0, "one thread per digit needed");

@@ -49,6 +49,18 @@ struct PassShape {
     static constexpr int TILE = BLOCK * ITEMS;
 };
 
+// Arrays below SMALL_PASS_N keys use 512-thread tiles: the large tiles of
+// PassShape would leave too few workgroups to fill the GPU.
+constexpr size_t SMALL_PASS_N = size_t(1) << 24;
+template <class KD, int PL>
+struct SmallPassShape {
+    using U = typename KD::U;
+    using V = typename PayloadT<PL>::type;
+    static constexpr int BLOCK = 512;
+    static constexpr int ITEMS = (sizeof(U) + (PL == PL_NONE ? 0 : sizeof(V))) > 8 ? 8 : 16;
+    static constexpr int TILE = BLOCK * ITEMS;
+};
+
 // largest array one workgroup sorts in LDS (<= 64 KiB of LDS)
 template <class KD, int PL>
 struct BlockShape {
@@ -66,9 +78,13 @@ struct SortPlan {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
     static constexpr int P = KD::PASSES;
-    static constexpr int TILE = PassShape<KD, PL>::TILE;
     static bool small(size_t n) { return n <= size_t(BlockShape<KD, PL>::MAXN); }
-    static size_t tiles(size_t n) { return (n + TILE - 1) / TILE; }
+    static bool small_tiles(size_t n) { return n < SMALL_PASS_N; }
+    static size_t tile(size_t n)
+    {
+        return small_tiles(n) ? size_t(SmallPassShape<KD, PL>::TILE) : size_t(PassShape<KD, PL>::TILE);
+    }
+    static size_t tiles(size_t n) { return (n + tile(n) - 1) / tile(n); }
     // persistent workgroups: contiguous ranges of whole tiles
     static uint32_t range_tiles(size_t n) { return uint32_t((tiles(n) + MAX_RANGES - 1) / MAX_RANGES); }
     static uint32_t ranges(size_t n) { return uint32_t((tiles(n) + range_tiles(n) - 1) / range_tiles(n)); }
@@ -80,6 +96,56 @@ struct SortPlan {
         return align256(n * sizeof(U)) + align256(n * sizeof(V));
     }
 };
+
+// Enqueue one 8-bit LSD pass over n keys (digit at `shift`): upsweep ->
+// range_scan -> downsweep, src -> dst, with the tile shape SortPlan picks
+// for n.  PL_IDX: `first` generates the indices, `last` writes them to
+// idx_out.
+template <class KD, int PL, class PS>
+int launch_downsweep(const PassArgs &a, uint32_t ranges, bool nan_last, hipStream_t s)
+{
+    KernelTimer t("downsweep", s);
+    if (nan_last)
+        hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, true>), dim3(ranges), dim3(PS::BLOCK), 0,
+                           s, a);
+    else
+        hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, false>), dim3(ranges), dim3(PS::BLOCK),
+                           0, s, a);
+    XSS_HIP_CHECK(hipGetLastError());
+    return XSS_OK;
+}
+
+template <class KD, int PL>
+int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *dst_k, void *dst_v,
+                     uint64_t *idx_out, size_t n, int shift, uint64_t flip, bool nan_last, bool first, bool last,
+                     bool canon_nan, hipStream_t s)
+{
+    using Plan = SortPlan<KD, PL>;
+    const uint32_t nn = uint32_t(n);
+    const uint32_t rtiles = Plan::range_tiles(n), ranges = Plan::ranges(n);
+    UpArgs u{};
+    u.keys = src_k; u.n = nn; u.range_keys = uint32_t(rtiles * Plan::tile(n)); u.shift = uint32_t(shift);
+    u.nan_last = nan_last; u.flip = flip; u.rsum = ws->rsum(); u.ranges = ranges;
+    {
+        KernelTimer t("upsweep", s);
+        hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    ScanArgs sc{};
+    sc.rsum = ws->rsum(); sc.rpre = ws->rpre(); sc.total = ws->total(); sc.ranges = ranges;
+    {
+        KernelTimer t("range_scan", s);
+        hipLaunchKernelGGL(range_scan_kernel<0>, dim3(256), dim3(SCAN_BLOCK), 0, s, sc);
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    PassArgs a{};
+    a.keys_in = src_k; a.vals_in = src_v; a.keys_out = dst_k; a.vals_out = dst_v; a.idx_out = idx_out;
+    a.n = nn; a.shift = uint32_t(shift); a.flip = flip; a.nan_last = nan_last; a.range_tiles = rtiles;
+    a.ranges = ranges; a.rpre = ws->rpre(); a.total = ws->total();
+    a.first = first; a.last = last; a.canon_nan = canon_nan;
+    if (Plan::small_tiles(n)) return launch_downsweep<KD, PL, SmallPassShape<KD, PL>>(a, ranges, nan_last, s);
+    return launch_downsweep<KD, PL, PassShape<KD, PL>>(a, ranges, nan_last, s);
+}
 
 // Enqueue one sort of n keys (in place when keys_in == keys_out).
 // `scratch` must hold SortPlan::scratch_bytes(n).
@@ -115,10 +181,6 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
         return XSS_OK;
     }
     constexpr int P = KD::PASSES;
-    using PS = PassShape<KD, PL>;
-    const uint32_t nn = uint32_t(n);
-    const uint32_t rtiles = Plan::range_tiles(n), ranges = Plan::ranges(n);
-
     // ping-pong buffers
     char *k0 = scratch, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
     if (PL == PL_IDX) {
@@ -132,48 +194,19 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
     }
     const void *src_k = keys_in, *src_v = vals_in;
     for (int p = 0; p < P; ++p) {
-        UpArgs u{};
-        u.keys = src_k; u.n = nn; u.range_keys = rtiles * PS::TILE; u.shift = 8 * p; u.nan_last = nan_last;
-        u.flip = flip; u.rsum = ws->rsum(); u.ranges = ranges;
-        {
-            KernelTimer t("upsweep", s);
-            hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
-        }
-        XSS_HIP_CHECK(hipGetLastError());
-        ScanArgs sc{};
-        sc.rsum = ws->rsum(); sc.rpre = ws->rpre(); sc.total = ws->total(); sc.ranges = ranges;
-        {
-            KernelTimer t("range_scan", s);
-            hipLaunchKernelGGL(range_scan_kernel<0>, dim3(256), dim3(SCAN_BLOCK), 0, s, sc);
-        }
-        XSS_HIP_CHECK(hipGetLastError());
-        PassArgs a{};
-        a.keys_in = src_k; a.vals_in = src_v; a.idx_out = idx_out; a.n = nn; a.shift = 8 * p;
-        a.flip = flip; a.nan_last = nan_last; a.range_tiles = rtiles; a.ranges = ranges;
-        a.rpre = ws->rpre(); a.total = ws->total();
-        a.first = (p == 0); a.last = (p == P - 1);
-        a.canon_nan = (p == P - 1) && canon && nan_last;
+        void *dk, *dv;
         if (PL == PL_IDX) {
-            if (p < P - 1) {
-                a.keys_out = (p % 2 == 0) ? k0 : k1;
-                a.vals_out = (p % 2 == 0) ? v0 : v1;
-            }
+            dk = p == P - 1 ? nullptr : (p % 2 == 0) ? k0 : k1;
+            dv = p == P - 1 ? nullptr : (p % 2 == 0) ? v0 : v1;
         } else {
-            a.keys_out = (p % 2 == 0) ? static_cast<void *>(k0) : keys_out;
-            a.vals_out = (p % 2 == 0) ? static_cast<void *>(v0) : vals_out;
+            dk = (p % 2 == 0) ? static_cast<void *>(k0) : keys_out;
+            dv = (p % 2 == 0) ? static_cast<void *>(v0) : vals_out;
         }
-        {
-            KernelTimer t("downsweep", s);
-            if (nan_last)
-                hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, true>), dim3(ranges),
-                                   dim3(PS::BLOCK), 0, s, a);
-            else
-                hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, false>), dim3(ranges),
-                                   dim3(PS::BLOCK), 0, s, a);
-        }
-        XSS_HIP_CHECK(hipGetLastError());
-        src_k = a.keys_out;
-        src_v = a.vals_out;
+        const int rc = enqueue_lsd_pass<KD, PL>(ws, src_k, src_v, dk, dv, idx_out, n, 8 * p, flip, nan_last, p == 0,
+                                                p == P - 1, (p == P - 1) && canon && nan_last, s);
+        if (rc) return rc;
+        src_k = dk;
+        src_v = dv;
     }
     return XSS_OK;
 }

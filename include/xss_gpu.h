@@ -67,9 +67,14 @@ XSS_API size_t xss_gpu_workspace_bytes(void);
  * and clears the record.  No counterpart in the reference. */
 XSS_API int xss_gpu_timing_enable(int on);
 /* Runtime tuning switches (no counterpart in the reference):
- *   "hybrid_min_n"  qsort of 4- and 8-byte keys sorts arrays of at least
- *                   this many keys MSD-first (two passes over HBM, then every
- *                   bucket of the top 16 bits in LDS); default 2^25. */
+ *   "hybrid_min_n"  qsort / keyvalue_qsort / argsort of 4- and 8-byte
+ *                   keys sort arrays of at least this many keys MSD-first
+ *                   (two passes over HBM, then every bucket of the top 16
+ *                   bits in LDS); default 2^25.
+ *   "hybrid_check"  1 (default): before an MSD-first sort, a sample of 4096
+ *                   keys (one small kernel and one wait for the stream; not
+ *                   while the stream is being captured) sends skewed or
+ *                   duplicate-heavy data to the LSD path instead; 0: no check. */
 XSS_API int xss_gpu_set_tuning(const char *name, uint64_t value);
 XSS_API int xss_gpu_timing_collect(int max_names, const char **names, double *total_ms, uint64_t *counts);
 

@@ -23,6 +23,7 @@
 // step is stable anyway.
 #pragma once
 #include <atomic>
+#include <vector>
 
 #include "xss_sort.hpp"
 
@@ -65,6 +66,7 @@ struct HybArgs {
     uint32_t *lstart;           // [lcap] the large buckets
     uint32_t *lsize;
     uint32_t lcap;
+    uint32_t cap;               // largest bucket the LDS sort takes
     HybTable tb;                // ranges of pass B (segments: top buckets)
     HybTable td;                // ranges of the low passes (segments: large buckets)
     uint64_t rk_b;              // keys per range
@@ -142,9 +144,9 @@ __global__ __launch_bounds__(256) void hyb_buckets_kernel(HybArgs a)
     const uint32_t b = s * 256 + d;
     a.bstart[b] = start;
     a.bsize[b] = size;
-    if (size > HYB_CAP) {
+    if (size > a.cap) {
         const uint32_t i = atomicAdd(a.nlarge, 1u);
-        if (i < a.lcap) {  // lcap >= n / (HYB_CAP + 1): always true
+        if (i < a.lcap) {  // lcap >= n / (cap + 1): always true
             a.lstart[i] = start;
             a.lsize[i] = size;
         }
@@ -186,30 +188,59 @@ __global__ __launch_bounds__(1024) void hyb_large_ranges_kernel(HybArgs a)
     }
 }
 
-// one workgroup per bucket of at most HYB_CAP keys: LSD radix by the low
-// P - 2 digits in LDS (the top two are the bucket's), written back in place
-template <class KD, bool NL>
-__global__ __launch_bounds__(HYB_BLOCK) XSS_HYB_OCC void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize,
-                                                              void *keys_v, uint64_t flip64, uint32_t canon)
+// Shape of the per-bucket LDS sort: 512 threads, up to ITEMS keys (and
+// payloads) each.  4-byte keys only: compiled for 8 waves per SIMD (its
+// SGPRs alone would cap it at 7).
+template <class KD, int PL>
+struct HybShape {
+    static constexpr int BYTES = ScatterClass<KD, PL>::bytes;
+    static constexpr int ITEMS = BYTES > 8 ? 8 : HYB_ITEMS;
+    static constexpr uint32_t CAP = uint32_t(HYB_BLOCK) * ITEMS;
+    static constexpr int WAVES = BYTES == 4 ? XSS_HYB_WAVES : 1;
+};
+
+// One workgroup per bucket of at most CAP keys: LSD radix by the low P - 2
+// digits in LDS (the top two are the bucket's), written back in place; the
+// payload (values, or argsort's 32-bit indices) moves with its key.
+// PL_IDX: the sorted indices go to idx_out (64-bit) at the bucket's place.
+template <class KD, int PL, bool NL>
+__global__ __launch_bounds__(HYB_BLOCK) __attribute__((amdgpu_waves_per_eu(HybShape<KD, PL>::WAVES)))
+void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_v, void *vals_v, uint64_t *idx_out,
+                      uint64_t flip64, uint32_t canon)
 {
     using U = typename KD::U;
+    using V = typename PayloadT<PL>::type;
+    constexpr bool HAS_PL = PL != PL_NONE;
     constexpr int P = KD::PASSES;
     constexpr int WAVES = HYB_BLOCK / 64;
-    __shared__ U s_keys[HYB_CAP];
+    constexpr int ITEMS = HybShape<KD, PL>::ITEMS;
+    constexpr uint32_t CAP = HybShape<KD, PL>::CAP;
+    __shared__ U s_keys[CAP];
+    __shared__ V s_vals[HAS_PL ? CAP : 1];
     __shared__ uint32_t s_off[WAVES * 256];
     __shared__ uint32_t s_wsum[WAVES];
     const uint32_t b = blockIdx.x;
     const uint32_t m = bsize[b];
-    if (m > HYB_CAP) return;  // a large bucket: the low segmented passes sort it
-    U *keys = static_cast<U *>(keys_v) + bstart[b];
+    if (m > CAP) return;  // a large bucket: the low segmented passes sort it
+    const uint32_t st = bstart[b];
+    U *keys = static_cast<U *>(keys_v) + st;
+    V *vals = HAS_PL ? static_cast<V *>(vals_v) + st : nullptr;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     auto finish = [&](U x) {
         if constexpr (KD::KIND == Kind::Float)
             if (canon && KD::is_nan(x)) x = KD::QNAN;
         return x;
     };
+    auto put = [&](uint32_t idx, U x, V v) {  // final key (and payload) at bucket slot idx
+        if constexpr (PL == PL_IDX) {
+            idx_out[st + idx] = uint64_t(v);
+        } else {
+            keys[idx] = finish(x);
+            if constexpr (HAS_PL) vals[idx] = v;
+        }
+    };
     if (m < 2) {
-        if (m == 1 && tid == 0) keys[0] = finish(keys[0]);
+        if (m == 1 && tid == 0) put(0, keys[0], HAS_PL ? vals[0] : V(0));
         return;
     }
     const U flip = U(flip64);
@@ -217,11 +248,14 @@ __global__ __launch_bounds__(HYB_BLOCK) XSS_HYB_OCC void hyb_local_kernel(const 
     // wave w holding keys [w * 64 * it, (w + 1) * 64 * it)
     const uint32_t it = (m + HYB_BLOCK - 1) / HYB_BLOCK;
     const uint32_t lbase = wave * 64 * it + lane;
-    U key[HYB_ITEMS];
+    U key[ITEMS];
+    V val[ITEMS];
 #pragma unroll
-    for (int i = 0; i < HYB_ITEMS; ++i) {
+    for (int i = 0; i < ITEMS; ++i) {
         const uint32_t idx = lbase + i * 64;
-        key[i] = (uint32_t(i) < it && idx < m) ? keys[idx] : U(0);
+        const bool ok = uint32_t(i) < it && idx < m;
+        key[i] = ok ? keys[idx] : U(0);
+        if constexpr (HAS_PL) val[i] = ok ? vals[idx] : V(0);
     }
 #pragma unroll 1
     for (int p = 0; p < P - 2; ++p) {
@@ -233,10 +267,10 @@ __global__ __launch_bounds__(HYB_BLOCK) XSS_HYB_OCC void hyb_local_kernel(const 
         // LDS atomic by the lowest of them, so successive items do not wait
         // on each other's counter update.  Padding takes digit 255 and comes
         // after every key.
-        uint32_t rd[HYB_ITEMS];
+        uint32_t rd[ITEMS];
         uint32_t *wc = s_off + wave * 256;
 #pragma unroll
-        for (int i = 0; i < HYB_ITEMS; ++i) {
+        for (int i = 0; i < ITEMS; ++i) {
             if (uint32_t(i) < it) {
                 const uint32_t idx = lbase + i * 64;
                 const uint32_t d = idx < m ? KD::digit(key[i], flip, NL, shift) : 255u;
@@ -264,41 +298,50 @@ __global__ __launch_bounds__(HYB_BLOCK) XSS_HYB_OCC void hyb_local_kernel(const 
         if (lane == 63) s_wsum[wave] = incl;
         __syncthreads();
         if (tid < 256) {
-            uint32_t st = incl - tcount;
-            for (uint32_t w = 0; w < wave; ++w) st += s_wsum[w];
+            uint32_t sw = incl - tcount;
+            for (uint32_t w = 0; w < wave; ++w) sw += s_wsum[w];
 #pragma unroll
             for (int w = 0; w < WAVES; ++w) {
                 const uint32_t c = s_off[w * 256 + tid];
-                s_off[w * 256 + tid] = st;
-                st += c;
+                s_off[w * 256 + tid] = sw;
+                sw += c;
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < HYB_ITEMS; ++i) {
+        for (int i = 0; i < ITEMS; ++i) {
             if (uint32_t(i) < it) {
                 const uint32_t d = rd[i] >> 16;
-                s_keys[wc[d] + (rd[i] & 0xffffu)] = key[i];
+                const uint32_t slot = wc[d] + (rd[i] & 0xffffu);
+                s_keys[slot] = key[i];
+                if constexpr (HAS_PL) s_vals[slot] = val[i];
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < HYB_ITEMS; ++i)
-            if (uint32_t(i) < it) key[i] = s_keys[lbase + i * 64];
+        for (int i = 0; i < ITEMS; ++i) {
+            if (uint32_t(i) < it) {
+                key[i] = s_keys[lbase + i * 64];
+                if constexpr (HAS_PL) val[i] = s_vals[lbase + i * 64];
+            }
+        }
     }
 #pragma unroll
-    for (int i = 0; i < HYB_ITEMS; ++i) {
+    for (int i = 0; i < ITEMS; ++i) {
         const uint32_t idx = lbase + i * 64;
-        if (uint32_t(i) < it && idx < m) keys[idx] = finish(key[i]);
+        if (uint32_t(i) < it && idx < m) put(idx, key[i], HAS_PL ? val[i] : V(0));
     }
 }
 
-template <class KD>
+template <class KD, int PL>
 struct HybPlan {
     using U = typename KD::U;
+    using V = typename PayloadT<PL>::type;
+    static constexpr bool HAS_PL = PL != PL_NONE;
+    static constexpr uint32_t CAP = HybShape<KD, PL>::CAP;
     // tile shape of the segmented passes (8K-key tiles for 4-byte keys
     // measured faster here than the one-array pass's 16K)
-    using SS = SegShape<KD, PL_NONE>;
+    using SS = SegShape<KD, PL>;
     static constexpr uint64_t SEG_TILE = uint64_t(SS::BLOCK) * SS::ITEMS;
     // pass B's table must fit the workspace's [256][MAX_RANGES] range tables
     static uint64_t rk_b(size_t n)
@@ -308,27 +351,40 @@ struct HybPlan {
     }
     static uint32_t cap_b(size_t n) { return uint32_t((n + rk_b(n) - 1) / rk_b(n) + 256); }
     static uint64_t rk_d(size_t) { return 8 * SEG_TILE; }
-    static uint32_t lcap(size_t n) { return uint32_t(std::min<size_t>(HYB_BUCKETS, n / (HYB_CAP + 1) + 1)); }
+    static uint32_t lcap(size_t n) { return uint32_t(std::min<size_t>(HYB_BUCKETS, n / (CAP + 1) + 1)); }
     static uint32_t cap_d(size_t n) { return uint32_t((n + rk_d(n) - 1) / rk_d(n) + lcap(n)); }
     static size_t table_bytes(uint32_t cap) { return 3 * align256(size_t(cap) * 8) + 2 * align256(size_t(cap) * 4); }
+    // key / payload buffers: keys-only and key-value sorts ping-pong with one
+    // scratch pair; argsort (read-only keys) runs between two scratch pairs
+    static size_t data_bytes(size_t n)
+    {
+        const size_t pair = align256(n * sizeof(U)) + (HAS_PL ? align256(n * sizeof(V)) : 0);
+        return PL == PL_IDX ? 2 * pair : pair;
+    }
     static size_t scratch_bytes(size_t n)
     {
-        return align256(n * sizeof(U)) + align256(256 * 256 * 4) + align256(257 * 8) + 2 * align256(HYB_BUCKETS * 4) +
+        return data_bytes(n) + align256(256 * 256 * 4) + align256(257 * 8) + 2 * align256(HYB_BUCKETS * 4) +
                align256(16) + table_bytes(cap_b(n)) + 2 * align256(size_t(lcap(n)) * 4) + table_bytes(cap_d(n)) +
                2 * align256(size_t(256) * cap_d(n) * 4) + align256(size_t(lcap(n)) * 256 * 4);
     }
 };
 
-template <class KD>
-int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasnan, bool desc, hipStream_t s)
+// MSD-first sort.  PL_NONE / PL_V32 / PL_V64: keys (and vals) sorted in
+// place; PL_IDX: keys read-only, the sorted order's indices to idx_out.
+template <class KD, int PL>
+int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_t *idx_out, size_t n, bool hasnan,
+                   bool desc, bool canon, hipStream_t s)
 {
     using U = typename KD::U;
-    using HP = HybPlan<KD>;
+    using V = typename PayloadT<PL>::type;
+    using HP = HybPlan<KD, PL>;
     using SS = typename HP::SS;
+    constexpr bool HAS_PL = PL != PL_NONE;
     constexpr int P = KD::PASSES;
     static_assert(P >= 4, "two top digits and at least two low digits");
     const uint64_t flip = desc ? ~0ull : 0ull;
     const bool nan_last = (KD::KIND == Kind::Float) && hasnan;
+    const bool canon_nan = canon && nan_last && PL == PL_NONE;
     const uint32_t nn = uint32_t(n);
     const uint32_t cap_b = HP::cap_b(n), cap_d = HP::cap_d(n), lcap = HP::lcap(n);
 
@@ -339,7 +395,14 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasn
         p += align256(bytes);
         return q;
     };
-    char *k0 = take(n * sizeof(U));
+    // X: where the second-digit pass writes and the buckets are sorted; Y: the other buffer
+    void *xk = keys, *xv = vals;
+    void *yk = take(n * sizeof(U));
+    void *yv = HAS_PL ? static_cast<void *>(take(n * sizeof(V))) : nullptr;
+    if (PL == PL_IDX) {
+        xk = take(n * sizeof(U));
+        xv = take(n * sizeof(V));
+    }
     uint32_t *segtot = reinterpret_cast<uint32_t *>(take(256 * 256 * 4));
     HybArgs h{};
     h.seg_off = reinterpret_cast<uint64_t *>(take(257 * 8));
@@ -366,13 +429,14 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasn
     h.segtot = segtot;
     h.nlarge = ctr;
     h.lcap = lcap;
+    h.cap = HP::CAP;
     h.rk_b = HP::rk_b(n);
     h.rk_d = HP::rk_d(n);
 
-    // pass A: top digit over the whole array (keys -> k0)
+    // pass A: top digit over the whole array (keys -> Y; argsort: indices generated)
     {
-        const int rc = enqueue_lsd_pass<KD, PL_NONE>(ws, keys, nullptr, k0, nullptr, nullptr, n, 8 * (P - 1), flip,
-                                                     nan_last, true, false, false, s);
+        const int rc = enqueue_lsd_pass<KD, PL>(ws, keys, vals, yk, yv, nullptr, n, 8 * (P - 1), flip, nan_last,
+                                                true, false, false, s);
         if (rc) return rc;
     }
     {
@@ -382,12 +446,13 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasn
     XSS_HIP_CHECK(hipGetLastError());
     // one segmented pass over a device-built table (src -> dst)
     auto seg_pass = [&](const char *const names[3], const HybTable &t, uint32_t *rsum, uint32_t *rpre,
-                        uint32_t *stot, int shift, const void *src, void *dst, bool canon) -> int {
+                        uint32_t *stot, int shift, const void *sk, const void *sv, void *dk, void *dv,
+                        bool last) -> int {
         // the kernels walk the table's ranges with a bounded grid: a table
         // that turns out (nearly) empty costs few workgroups
         const uint32_t grid = std::min<uint32_t>(t.cap, MAX_RANGES);
         UpArgs u{};
-        u.keys = src; u.n = nn; u.shift = uint32_t(shift); u.nan_last = nan_last; u.flip = flip;
+        u.keys = sk; u.n = nn; u.shift = uint32_t(shift); u.nan_last = nan_last; u.flip = flip;
         u.rsum = rsum; u.ranges = t.cap; u.range_begin = t.begin; u.range_end = t.end; u.nr_dev = t.nr;
         {
             KernelTimer kt(names[0], s);
@@ -401,23 +466,24 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasn
         }
         XSS_HIP_CHECK(hipGetLastError());
         PassArgs a{};
-        a.keys_in = src; a.keys_out = dst; a.n = nn; a.shift = uint32_t(shift); a.flip = flip;
-        a.nan_last = nan_last; a.canon_nan = canon && nan_last;
+        a.keys_in = sk; a.vals_in = sv; a.keys_out = dk; a.vals_out = dv; a.idx_out = idx_out; a.n = nn;
+        a.shift = uint32_t(shift); a.flip = flip; a.nan_last = nan_last; a.first = false; a.last = last;
+        a.canon_nan = last && canon_nan;
         SegRanges r{t.begin, t.end, t.sbase, t.bseg, rpre, stot, 0, t.cap, t.nr};
         KernelTimer kt(names[2], s);
         if (nan_last)
-            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL_NONE, SS::BLOCK, SS::ITEMS, true>), dim3(grid),
+            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, SS::BLOCK, SS::ITEMS, true>), dim3(grid),
                                dim3(SS::BLOCK), 0, s, a, r);
         else
-            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL_NONE, SS::BLOCK, SS::ITEMS, false>), dim3(grid),
+            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, SS::BLOCK, SS::ITEMS, false>), dim3(grid),
                                dim3(SS::BLOCK), 0, s, a, r);
         XSS_HIP_CHECK(hipGetLastError());
         return XSS_OK;
     };
-    // pass B: second digit inside every top bucket (k0 -> keys)
+    // pass B: second digit inside every top bucket (Y -> X)
     static const char *const msd_names[3] = {"hyb_msd_upsweep", "hyb_msd_scan", "hyb_msd_downsweep"};
     static const char *const low_names[3] = {"hyb_large_upsweep", "hyb_large_scan", "hyb_large_downsweep"};
-    int rc = seg_pass(msd_names, h.tb, ws->rsum(), ws->rpre(), segtot, 8 * (P - 2), k0, keys, false);
+    int rc = seg_pass(msd_names, h.tb, ws->rsum(), ws->rpre(), segtot, 8 * (P - 2), yk, yv, xk, xv, false);
     if (rc) return rc;
     {
         KernelTimer t("hyb_tables", s);
@@ -427,11 +493,11 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasn
     {
         KernelTimer t("hyb_local", s);
         if (nan_last)
-            hipLaunchKernelGGL((hyb_local_kernel<KD, true>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart,
-                               h.bsize, keys, flip, uint32_t(nan_last));
+            hipLaunchKernelGGL((hyb_local_kernel<KD, PL, true>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart,
+                               h.bsize, xk, xv, idx_out, flip, uint32_t(canon_nan));
         else
-            hipLaunchKernelGGL((hyb_local_kernel<KD, false>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart,
-                               h.bsize, keys, flip, 0u);
+            hipLaunchKernelGGL((hyb_local_kernel<KD, PL, false>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart,
+                               h.bsize, xk, xv, idx_out, flip, 0u);
     }
     XSS_HIP_CHECK(hipGetLastError());
     {
@@ -439,14 +505,14 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasn
         hipLaunchKernelGGL(hyb_large_ranges_kernel<0>, dim3(1), dim3(1024), 0, s, h);
     }
     XSS_HIP_CHECK(hipGetLastError());
-    // low digits of the large buckets (keys -> k0 -> keys; P - 2 is even)
-    const void *src = keys;
-    void *dst = k0;
+    // low digits of the large buckets (X -> Y -> X ...; P - 2 is even; the
+    // last pass of argsort writes the indices)
+    void *sk = xk, *sv = xv, *dk = yk, *dv = yv;
     for (int q = 0; q < P - 2; ++q) {
-        rc = seg_pass(low_names, h.td, rsum_d, rpre_d, segtot_d, 8 * q, src, dst, q == P - 3);
+        rc = seg_pass(low_names, h.td, rsum_d, rpre_d, segtot_d, 8 * q, sk, sv, dk, dv, q == P - 3);
         if (rc) return rc;
-        src = dst;
-        dst = (dst == k0) ? keys : static_cast<void *>(k0);
+        std::swap(sk, dk);
+        std::swap(sv, dv);
     }
     return XSS_OK;
 }
@@ -455,24 +521,88 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, size_t n, bool hasn
 
 namespace xss {
 
-// arrays of at least this many keys take the MSD-first path (xss_gpu_set_tuning)
+// arrays of at least this many keys take the MSD-first path, and whether the
+// sampled spread check runs first (xss_gpu_set_tuning)
 std::atomic<uint64_t> &hybrid_min_n();
+std::atomic<uint64_t> &hybrid_check();
 
-// x86simdsort::qsort on the device: MSD-first for large arrays of 4- and
-// 8-byte keys, LSD otherwise
+constexpr uint32_t HYB_SAMPLES = 4096;
+
+// top 16 bits of the order keys of HYB_SAMPLES evenly spaced keys
 template <class KD>
-int run_qsort(void *keys, size_t n, bool hasnan, bool desc, hipStream_t s)
+__global__ __launch_bounds__(256) void hyb_sample_kernel(const void *keys_v, uint64_t n, uint64_t flip64,
+                                                         uint32_t nan_last, uint16_t *out)
+{
+    using U = typename KD::U;
+    const U *keys = static_cast<const U *>(keys_v);
+    for (uint32_t i = threadIdx.x; i < HYB_SAMPLES; i += 256) {
+        const uint64_t pos = (uint64_t(i) * n) / HYB_SAMPLES;
+        const U o = KD::ord(keys[pos], U(flip64), nan_last != 0);
+        out[i] = uint16_t(o >> (8 * sizeof(U) - 16));
+    }
+}
+
+// Whether the top 16 bits spread the keys over buckets small enough for
+// the LDS sort: in a sample of HYB_SAMPLES keys, samples that share their
+// top 16 bits with another sample point at buckets of >= ~2n/HYB_SAMPLES
+// keys (far above an LDS bucket at these sizes).  Skewed or
+// duplicate-heavy data -- most samples in such buckets -- sorts faster on
+// the LSD path.  Costs one small kernel and one wait for the stream; a
+// stream being captured into a graph cannot wait, and takes the MSD-first
+// path unconditionally.
+template <class KD>
+bool hybrid_spread_ok(Workspace *ws, const void *keys, size_t n, bool nan_last, uint64_t flip, hipStream_t s)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return true;
+    static thread_local uint16_t *host = nullptr;
+    if (!host && hipHostMalloc(reinterpret_cast<void **>(&host), HYB_SAMPLES * 2) != hipSuccess) {
+        host = nullptr;
+        return true;
+    }
+    uint16_t *dev = reinterpret_cast<uint16_t *>(ws->rpre());  // free until the sort's first scan
+    hipLaunchKernelGGL(hyb_sample_kernel<KD>, dim3(1), dim3(256), 0, s, keys, uint64_t(n), flip,
+                       uint32_t(nan_last), dev);
+    if (hipGetLastError() != hipSuccess) return true;
+    if (hipMemcpyAsync(host, dev, HYB_SAMPLES * 2, hipMemcpyDeviceToHost, s) != hipSuccess) return true;
+    if (hipStreamSynchronize(s) != hipSuccess) return true;
+    static thread_local std::vector<uint8_t> seen(65536, 0);
+    uint32_t shared = 0;
+    for (uint32_t i = 0; i < HYB_SAMPLES; ++i) {
+        uint8_t &c = seen[host[i]];
+        shared += c == 1 ? 2u : c > 1 ? 1u : 0u;  // the first repeat counts both samples
+        c = uint8_t(c < 2 ? c + 1 : 2);
+    }
+    for (uint32_t i = 0; i < HYB_SAMPLES; ++i) seen[host[i]] = 0;
+    return shared * 5 <= HYB_SAMPLES;  // at most 20% of the samples in shared buckets
+}
+
+// qsort / keyvalue_qsort / argsort on the device: MSD-first for large arrays
+// of 4- and 8-byte keys, LSD otherwise (same arguments as run_sort)
+template <class KD, int PL>
+int run_sort_auto(const void *keys_in, void *keys_out, const void *vals_in, void *vals_out, uint64_t *idx_out,
+                  size_t n, bool hasnan, bool desc, bool canon, hipStream_t s)
 {
     if constexpr (sizeof(typename KD::U) >= 4) {
-        if (keys && n >= std::max<uint64_t>(hybrid_min_n().load(), 2) && n <= MAX_N) {
+        const bool in_place = PL == PL_IDX || (keys_in == keys_out && vals_in == vals_out);
+        const bool args_ok = keys_in && (PL == PL_IDX ? idx_out != nullptr : keys_out != nullptr) &&
+                             (PL == PL_NONE || PL == PL_IDX || (vals_in && vals_out));
+        if (in_place && args_ok && n >= std::max<uint64_t>(hybrid_min_n().load(), 2) && n <= MAX_N) {
             Workspace *ws = nullptr;
             std::unique_lock<std::mutex> lock;
-            int rc = acquire_workspace(s, HybPlan<KD>::scratch_bytes(n), &ws, &lock);
+            int rc = acquire_workspace(s, HybPlan<KD, PL>::scratch_bytes(n), &ws, &lock);
             if (rc) return rc;
-            return enqueue_hybrid<KD>(ws, ws->data, keys, n, hasnan, desc, s);
+            const bool nan_last = (KD::KIND == Kind::Float) && hasnan;
+            const uint64_t flip = desc ? ~0ull : 0ull;
+            if (hybrid_check() == 0 || hybrid_spread_ok<KD>(ws, keys_in, n, nan_last, flip, s))
+                return enqueue_hybrid<KD, PL>(ws, ws->data, const_cast<void *>(keys_in),
+                                              const_cast<void *>(vals_in), idx_out, n, hasnan, desc, canon, s);
+            // skewed data: the LSD path (its scratch fits in the workspace just grown)
+            return enqueue_sort<KD, PL>(ws, ws->data, keys_in, keys_out, vals_in, vals_out, idx_out, n, hasnan,
+                                        desc, canon, s);
         }
     }
-    return run_sort<KD, PL_NONE>(keys, keys, nullptr, nullptr, nullptr, n, hasnan, desc, true, s);
+    return run_sort<KD, PL>(keys_in, keys_out, vals_in, vals_out, idx_out, n, hasnan, desc, canon, s);
 }
 
 }  // namespace xss

@@ -10,13 +10,14 @@ using namespace xss;
 #define XSS_DEF_SORT(tag, T, CT)                                                                      \
     extern "C" int xss_gpu_qsort_##tag(T *arr, size_t n, int hasnan, int descending, xss_stream_t s)  \
     {                                                                                                 \
-        return run_qsort<TypeKey<CT>::KD>(arr, n, hasnan != 0, descending != 0, s);                   \
+        return run_sort_auto<TypeKey<CT>::KD, PL_NONE>(arr, arr, nullptr, nullptr, nullptr, n, hasnan != 0,  \
+                                                        descending != 0, true, s);                    \
     }                                                                                                 \
     extern "C" int xss_gpu_argsort_##tag(const T *arr, uint64_t *arg, size_t n, int hasnan,           \
                                          int descending, xss_stream_t s)                              \
     {                                                                                                 \
-        return run_sort<TypeKey<CT>::KD, PL_IDX>(arr, nullptr, nullptr, nullptr, arg, n, hasnan != 0,  \
-                                                  descending != 0, false, s);                         \
+        return run_sort_auto<TypeKey<CT>::KD, PL_IDX>(arr, nullptr, nullptr, nullptr, arg, n, hasnan != 0, \
+                                                       descending != 0, false, s);                    \
     }
 
 XSS_DEF_SORT(u16, uint16_t, uint16_t)
@@ -35,6 +36,11 @@ std::atomic<uint64_t> &hybrid_min_n()
     static std::atomic<uint64_t> v{HYB_MIN_N};
     return v;
 }
+std::atomic<uint64_t> &hybrid_check()
+{
+    static std::atomic<uint64_t> v{1};
+    return v;
+}
 }  // namespace xss
 
 extern "C" int xss_gpu_set_tuning(const char *name, uint64_t value)
@@ -42,6 +48,10 @@ extern "C" int xss_gpu_set_tuning(const char *name, uint64_t value)
     if (!name) return set_error(XSS_ERR_INVALID, "null tuning name");
     if (std::string(name) == "hybrid_min_n") {
         hybrid_min_n().store(value);
+        return XSS_OK;
+    }
+    if (std::string(name) == "hybrid_check") {
+        hybrid_check().store(value);
         return XSS_OK;
     }
     return set_error(XSS_ERR_INVALID, "unknown tuning '%s'", name);

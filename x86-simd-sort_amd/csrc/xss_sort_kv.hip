@@ -2,7 +2,7 @@
 // (x86simdsort::keyvalue_qsort, lib/x86simdsort.h:46-52; the 36 key/value
 // pairs of lib/x86simdsort.cpp:280-293).  Values move as 32- or 64-bit
 // payloads; the value type matters only through its width.
-#include "xss_sort.hpp"
+#include "xss_hybrid.hpp"
 
 using namespace xss;
 
@@ -10,7 +10,7 @@ using namespace xss;
     extern "C" int xss_gpu_keyvalue_qsort_##ktag##_##vtag(K *key, V *val, size_t n, int hasnan,   \
                                                           int descending, xss_stream_t s)         \
     {                                                                                             \
-        return run_sort<TypeKey<K>::KD, payload_of<V>()>(key, key, val, val, nullptr, n,          \
+        return run_sort_auto<TypeKey<K>::KD, payload_of<V>()>(key, key, val, val, nullptr, n,     \
                                                          hasnan != 0, descending != 0, false, s); \
     }
 

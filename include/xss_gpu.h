@@ -74,7 +74,10 @@ XSS_API int xss_gpu_timing_enable(int on);
  *   "hybrid_check"  1 (default): before an MSD-first sort, a sample of 4096
  *                   keys (one small kernel and one wait for the stream; not
  *                   while the stream is being captured) sends skewed or
- *                   duplicate-heavy data to the LSD path instead; 0: no check. */
+ *                   duplicate-heavy data to the LSD path instead; 0: no check.
+ *   "count_min_n"   qsort of 16-bit keys counts keys per value (a counting
+ *                   sort: one read, one write) for arrays of at least this
+ *                   many keys; default 2^23. */
 XSS_API int xss_gpu_set_tuning(const char *name, uint64_t value);
 XSS_API int xss_gpu_timing_collect(int max_names, const char **names, double *total_ms, uint64_t *counts);
 

@@ -1,6 +1,7 @@
 """Throughput of every single-array workload on one GPU, device-resident.
 
     python x86-simd-sort_amd/bench_configs.py [--reps 3] [--only c1,c2,c3]
+        [--types u16,f32,...] [--recipes random,sorted,...]
 
 c1  qsort float32, N = 1,000,000, uniform in [-1e6, 1e6], ascending.
 c2  qsort and argsort, each type at 1 GiB of keys (2^28 32-bit, 2^27 64-bit,
@@ -18,6 +19,7 @@ on; the median of --reps is reported.  One JSON line per measurement with
 Gkeys/s and the fraction of the 8 TB/s HBM peak at the algorithmic bytes
 per key of the 8-bit LSD plan:
   sort      s_k (1 + 2P)
+  sort      2 s_k for 16-bit keys (the counting sort: one read, one write)
   argsort   s_k + (2 s_k + s_i) + 2 (P - 2)(s_k + s_i) + (s_k + s_i + s_o)
   kv sort   s_k + 2P (s_k + s_v)
   select    4 s_k (+ 2 s_v for key-value)
@@ -49,7 +51,7 @@ def passes(kb):
 def alg_bytes(op, kb, vb=0):
     P = passes(kb)
     if op == "qsort":
-        return kb * (1 + 2 * P)
+        return 2 * kb if kb == 2 else kb * (1 + 2 * P)
     if op == "argsort":
         si, so = 4, 8
         return kb + (2 * kb + si) + 2 * (P - 2) * (kb + si) + (kb + si + so)
@@ -147,12 +149,16 @@ def c1(torch, L, reps):
     emit("c1", "qsort", "f32", n, ms, alg_bytes("qsort", 4))
 
 
-def c2(torch, L, reps):
+def c2(torch, L, reps, types=None, recipes=None):
     for tag in xss_amd.TAGS:
+        if types and tag not in types:
+            continue
         kb = BYTES[tag]
         n = N_FULL[kb]
         idx = torch.empty(n, dtype=torch.int64, device="cuda")
         for recipe in RECIPES:
+            if recipes and recipe not in recipes:
+                continue
             if recipe == "nan" and tag not in ("f16", "f32", "f64"):
                 continue
             hasnan = int(recipe == "nan")
@@ -242,11 +248,18 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--only", default="c1,c2,c3")
+    ap.add_argument("--types", default="", help="c2: comma-separated key types (default all)")
+    ap.add_argument("--recipes", default="", help="c2: comma-separated recipes (default all)")
     args = ap.parse_args()
     import torch
     L = xss_amd.lib()
+    types = [t for t in args.types.split(",") if t]
+    recipes = [r for r in args.recipes.split(",") if r]
     for c in args.only.split(","):
-        {"c1": c1, "c2": c2, "c3": c3}[c](torch, L, args.reps)
+        if c == "c2":
+            c2(torch, L, args.reps, types, recipes)
+        else:
+            {"c1": c1, "c3": c3}[c](torch, L, args.reps)
 
 
 if __name__ == "__main__":

@@ -25,6 +25,7 @@
 #include <atomic>
 #include <vector>
 
+#include "xss_count.hpp"
 #include "xss_sort.hpp"
 
 namespace xss {
@@ -525,6 +526,7 @@ namespace xss {
 // sampled spread check runs first (xss_gpu_set_tuning)
 std::atomic<uint64_t> &hybrid_min_n();
 std::atomic<uint64_t> &hybrid_check();
+std::atomic<uint64_t> &count_min_n();  // 16-bit qsort: counting sort from this size
 
 constexpr uint32_t HYB_SAMPLES = 4096;
 
@@ -583,6 +585,19 @@ template <class KD, int PL>
 int run_sort_auto(const void *keys_in, void *keys_out, const void *vals_in, void *vals_out, uint64_t *idx_out,
                   size_t n, bool hasnan, bool desc, bool canon, hipStream_t s)
 {
+    if constexpr (sizeof(typename KD::U) == 2 && PL == PL_NONE) {
+        // 16-bit keys only: a counting sort (one read, one write).  Every
+        // NaN leaves it as the canonical NaN, so it serves qsort (canon).
+        if (keys_in && keys_in == keys_out && canon && (reinterpret_cast<uintptr_t>(keys_out) & 1u) == 0 &&
+            n >= std::max<uint64_t>(count_min_n().load(), 2) && n <= MAX_N) {
+            const uint32_t rows = cs_rows();
+            Workspace *ws = nullptr;
+            std::unique_lock<std::mutex> lock;
+            int rc = acquire_workspace(s, cs_scratch_bytes(rows), &ws, &lock);
+            if (rc) return rc;
+            return enqueue_count_sort<KD>(ws->data, keys_out, n, hasnan, desc, canon, rows, s);
+        }
+    }
     if constexpr (sizeof(typename KD::U) >= 4) {
         const bool in_place = PL == PL_IDX || (keys_in == keys_out && vals_in == vals_out);
         const bool args_ok = keys_in && (PL == PL_IDX ? idx_out != nullptr : keys_out != nullptr) &&

@@ -41,6 +41,11 @@ std::atomic<uint64_t> &hybrid_check()
     static std::atomic<uint64_t> v{1};
     return v;
 }
+std::atomic<uint64_t> &count_min_n()
+{
+    static std::atomic<uint64_t> v{CS_MIN_N};
+    return v;
+}
 }  // namespace xss
 
 extern "C" int xss_gpu_set_tuning(const char *name, uint64_t value)
@@ -52,6 +57,10 @@ extern "C" int xss_gpu_set_tuning(const char *name, uint64_t value)
     }
     if (std::string(name) == "hybrid_check") {
         hybrid_check().store(value);
+        return XSS_OK;
+    }
+    if (std::string(name) == "count_min_n") {
+        count_min_n().store(value);
         return XSS_OK;
     }
     return set_error(XSS_ERR_INVALID, "unknown tuning '%s'", name);

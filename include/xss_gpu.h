@@ -102,6 +102,37 @@ XSS_DECLARE_T(u64, uint64_t)
 XSS_DECLARE_T(i64, int64_t)
 XSS_DECLARE_T(f64, double)
 
+/*
+ * NumPy-style arg forms (x86simdsortStatic::argsort(T *, size_t *arg, size,
+ * hasnan, descending) and argselect(T *, size_t *arg, k, size, hasnan),
+ * src/x86simdsort-static-incl.h:31-36,43-46,92-97,108-113).  `arg` holds n
+ * indices into `arr` on entry (NumPy passes 0..n-1) and is reordered in place
+ * so that arr[arg[i]] is sorted, or has rank k at position k with smaller
+ * keys before it and larger after (argselect), with the NaN and descending
+ * rules of argsort / argselect.  `arr` is not modified.  Indices must be
+ * < n; an index >= n (undefined input for the reference) is read as index 0.
+ * The xss_gpu_ forms take device pointers and are stream-ordered; the
+ * xss_host_ forms take host pointers, stage them through the GPU and return
+ * when `arg` holds the result.
+ */
+#define XSS_DECLARE_ARGFORM(tag, T)                                                                     \
+    XSS_API int xss_gpu_argsort_arg_##tag(const T *arr, uint64_t *arg, size_t n, int hasnan, int descending,   \
+                                          xss_stream_t stream);                                         \
+    XSS_API int xss_gpu_argselect_arg_##tag(const T *arr, uint64_t *arg, size_t k, size_t n, int hasnan,       \
+                                            xss_stream_t stream);                                       \
+    XSS_API int xss_host_argsort_arg_##tag(const T *arr, size_t *arg, size_t n, int hasnan, int descending);   \
+    XSS_API int xss_host_argselect_arg_##tag(const T *arr, size_t *arg, size_t k, size_t n, int hasnan);
+
+XSS_DECLARE_ARGFORM(u16, uint16_t)
+XSS_DECLARE_ARGFORM(i16, int16_t)
+XSS_DECLARE_ARGFORM(f16, xss_f16_bits)
+XSS_DECLARE_ARGFORM(u32, uint32_t)
+XSS_DECLARE_ARGFORM(i32, int32_t)
+XSS_DECLARE_ARGFORM(f32, float)
+XSS_DECLARE_ARGFORM(u64, uint64_t)
+XSS_DECLARE_ARGFORM(i64, int64_t)
+XSS_DECLARE_ARGFORM(f64, double)
+
 #define XSS_DECLARE_KV(ktag, K, vtag, V)                                                           \
     XSS_API int xss_gpu_keyvalue_qsort_##ktag##_##vtag(K *key, V *val, size_t n, int hasnan,               \
                                                int descending, xss_stream_t stream);               \

@@ -28,8 +28,9 @@ def test_library_loads():
 def test_every_c_declaration_is_exported():
     pre = subprocess.run(["gcc", "-E", "-P", "-x", "c", os.path.join(INC, "xss_gpu.h")], capture_output=True,
                          text=True, check=True).stdout
-    names = set(re.findall(r"\b((?:xss_gpu_|keyvalue_qsort_)\w+)\s*\(", pre))
-    assert len(names) >= 174, len(names)
+    names = set(re.findall(r"\b((?:xss_gpu_|xss_host_|keyvalue_qsort_)\w+)\s*\(", pre))
+    assert len(names) >= 210, len(names)
+    assert {f"xss_host_argsort_arg_{t}" for t in T.TAGS} <= names
     missing = sorted(names - exported())
     assert not missing, missing
 
@@ -67,3 +68,29 @@ def test_exports_cover_the_reference_library():
     ref_syms = {ln.split()[-1] for ln in out.splitlines() if "x86simdsort" in ln}
     assert len(ref_syms) >= 150
     assert not sorted(ref_syms - exported())
+
+
+def test_static_api_resolves_to_library_symbols(tmp_path):
+    """include/x86simdsort-static-incl.h (the reference's x86simdsortStatic::
+    names, src/x86simdsort-static-incl.h:7-71): every function, including the
+    NumPy arg forms, compiles with g++ and links to symbols the library
+    defines."""
+    types = ["uint16_t", "int16_t", "float", "double", "uint32_t", "int32_t", "uint64_t", "int64_t"]
+    src = ['#include "x86simdsort-static-incl.h"', "void use() {"]
+    for t in types:
+        src += [f"{{ {t} *a = nullptr; size_t *g = nullptr; x86simdsortStatic::qsort(a, 0);",
+                f"  x86simdsortStatic::qselect(a, 0, 0); x86simdsortStatic::partial_qsort(a, 0, 0);",
+                f"  x86simdsortStatic::argsort(a, 0); x86simdsortStatic::argsort(a, g, 0, true, true);",
+                f"  x86simdsortStatic::argselect(a, 0, 0); x86simdsortStatic::argselect(a, g, 0, 0, true); }}"]
+    src += ["{ float *k = nullptr; uint64_t *v = nullptr; x86simdsortStatic::keyvalue_qsort(k, v, 0);",
+            "  x86simdsortStatic::keyvalue_select(k, v, 0, 0); x86simdsortStatic::keyvalue_partial_sort(k, v, 0, 0); }",
+            "}"]
+    f = tmp_path / "use_static.cpp"
+    f.write_text("\n".join(src))
+    obj = tmp_path / "use_static.o"
+    subprocess.run(["g++", "-std=c++17", "-c", "-I", INC, str(f), "-o", str(obj)], check=True)
+    und = subprocess.run(["nm", "-u", str(obj)], capture_output=True, text=True, check=True).stdout
+    need = {ln.split()[-1] for ln in und.splitlines() if "x86simdsort" in ln or "xss_" in ln}
+    assert {f"xss_host_argsort_arg_{t}" for t in ("u16", "i16", "f32", "f64", "u32", "i32", "u64", "i64")} <= need
+    missing = sorted(need - exported())
+    assert not missing, missing

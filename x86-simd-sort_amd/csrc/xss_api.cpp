@@ -78,6 +78,65 @@ void sync() { hip_ok(hipStreamSynchronize(stream()), "hipStreamSynchronize"); }
 
 }  // namespace
 
+namespace xss {
+int set_error(int code, const char *fmt, ...);
+}
+
+// Host-pointer NumPy-style arg forms: `arr` and `arg` staged to the device,
+// the xss_gpu_ form run, `arg` copied back.  Status codes as the C ABI.
+#define XSS_HOST_ARGFORM(tag, T)                                                                        \
+    extern "C" int xss_host_argsort_arg_##tag(const T *arr, size_t *arg, size_t n, int hasnan,          \
+                                              int descending)                                           \
+    {                                                                                                   \
+        try {                                                                                           \
+            if (n <= 1) return XSS_OK;                                                                  \
+            if (!arr || !arg) return xss::set_error(XSS_ERR_INVALID, "null pointer argument");         \
+            const size_t b = n * sizeof(T);                                                             \
+            char *d = device_buffer(align256(b) + n * 8);                                               \
+            uint64_t *di = reinterpret_cast<uint64_t *>(d + align256(b));                               \
+            h2d(d, arr, b);                                                                             \
+            h2d(di, arg, n * 8);                                                                        \
+            const int rc = xss_gpu_argsort_arg_##tag(reinterpret_cast<const T *>(d), di, n, hasnan,     \
+                                                     descending, stream());                             \
+            if (rc != XSS_OK) return rc;                                                                \
+            d2h(arg, di, n * 8);                                                                        \
+            sync();                                                                                     \
+            return XSS_OK;                                                                              \
+        } catch (const std::exception &e) {                                                             \
+            return xss::set_error(XSS_ERR_HIP, "%s", e.what());                                         \
+        }                                                                                               \
+    }                                                                                                   \
+    extern "C" int xss_host_argselect_arg_##tag(const T *arr, size_t *arg, size_t k, size_t n, int hasnan) \
+    {                                                                                                   \
+        try {                                                                                           \
+            if (n <= 1) return XSS_OK;                                                                  \
+            if (!arr || !arg) return xss::set_error(XSS_ERR_INVALID, "null pointer argument");         \
+            const size_t b = n * sizeof(T);                                                             \
+            char *d = device_buffer(align256(b) + n * 8);                                               \
+            uint64_t *di = reinterpret_cast<uint64_t *>(d + align256(b));                               \
+            h2d(d, arr, b);                                                                             \
+            h2d(di, arg, n * 8);                                                                        \
+            const int rc = xss_gpu_argselect_arg_##tag(reinterpret_cast<const T *>(d), di, k, n, hasnan, \
+                                                       stream());                                       \
+            if (rc != XSS_OK) return rc;                                                                \
+            d2h(arg, di, n * 8);                                                                        \
+            sync();                                                                                     \
+            return XSS_OK;                                                                              \
+        } catch (const std::exception &e) {                                                             \
+            return xss::set_error(XSS_ERR_HIP, "%s", e.what());                                         \
+        }                                                                                               \
+    }
+
+XSS_HOST_ARGFORM(u16, uint16_t)
+XSS_HOST_ARGFORM(i16, int16_t)
+XSS_HOST_ARGFORM(f16, xss_f16_bits)
+XSS_HOST_ARGFORM(u32, uint32_t)
+XSS_HOST_ARGFORM(i32, int32_t)
+XSS_HOST_ARGFORM(f32, float)
+XSS_HOST_ARGFORM(u64, uint64_t)
+XSS_HOST_ARGFORM(i64, int64_t)
+XSS_HOST_ARGFORM(f64, double)
+
 namespace x86simdsort {
 
 #define XSS_HOST_T(tag, T, CT)                                                                          \

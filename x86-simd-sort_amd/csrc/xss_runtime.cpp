@@ -83,6 +83,43 @@ int acquire_workspace(hipStream_t stream, size_t data_bytes, Workspace **out, st
     return XSS_OK;
 }
 
+int acquire_aux(hipStream_t stream, size_t bytes, char **out, std::unique_lock<std::mutex> *lock)
+{
+    int dev = 0;
+    XSS_HIP_CHECK(hipGetDevice(&dev));
+    Workspace *ws;
+    {
+        std::lock_guard<std::mutex> g(g_map_mu);
+        auto &slot = g_ws[{dev, stream}];
+        if (!slot) {
+            slot.reset(new Workspace());
+            slot->device = dev;
+            slot->stream = stream;
+        }
+        ws = slot.get();
+    }
+    std::unique_lock<std::mutex> l(ws->aux_mu);
+    if (bytes > ws->aux_cap) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        XSS_HIP_CHECK(hipStreamIsCapturing(stream, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return set_error(XSS_ERR_CAPTURE,
+                             "staging buffer must grow to %zu bytes during stream capture; "
+                             "run the call once before capturing",
+                             bytes);
+        XSS_HIP_CHECK(hipStreamSynchronize(stream));
+        if (ws->aux) XSS_HIP_CHECK(hipFree(ws->aux));
+        ws->aux = nullptr;
+        ws->aux_cap = 0;
+        const size_t cap = round_up(bytes + bytes / 4, 1 << 20);
+        XSS_HIP_CHECK(hipMalloc(&ws->aux, cap));
+        ws->aux_cap = cap;
+    }
+    *out = ws->aux;
+    *lock = std::move(l);
+    return XSS_OK;
+}
+
 namespace {
 struct TimedLaunch {
     std::string name;
@@ -171,6 +208,7 @@ int xss_gpu_release_workspaces(void)
     std::lock_guard<std::mutex> g(xss::g_map_mu);
     for (auto &kv : xss::g_ws) {
         xss::Workspace *ws = kv.second.get();
+        std::lock_guard<std::mutex> la(ws->aux_mu);
         std::lock_guard<std::mutex> l(ws->mu);
         int cur = 0;
         XSS_HIP_CHECK(hipGetDevice(&cur));
@@ -178,8 +216,9 @@ int xss_gpu_release_workspaces(void)
         XSS_HIP_CHECK(hipStreamSynchronize(ws->stream));
         if (ws->meta) XSS_HIP_CHECK(hipFree(ws->meta));
         if (ws->data) XSS_HIP_CHECK(hipFree(ws->data));
-        ws->meta = ws->data = nullptr;
-        ws->data_cap = 0;
+        if (ws->aux) XSS_HIP_CHECK(hipFree(ws->aux));
+        ws->meta = ws->data = ws->aux = nullptr;
+        ws->data_cap = ws->aux_cap = 0;
         XSS_HIP_CHECK(hipSetDevice(cur));
     }
     xss::g_ws.clear();
@@ -190,7 +229,8 @@ size_t xss_gpu_workspace_bytes(void)
 {
     std::lock_guard<std::mutex> g(xss::g_map_mu);
     size_t s = 0;
-    for (auto &kv : xss::g_ws) s += (kv.second->meta ? xss::META_BYTES : 0) + kv.second->data_cap;
+    for (auto &kv : xss::g_ws)
+        s += (kv.second->meta ? xss::META_BYTES : 0) + kv.second->data_cap + kv.second->aux_cap;
     return s;
 }
 

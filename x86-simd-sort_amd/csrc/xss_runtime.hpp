@@ -28,6 +28,9 @@ struct Workspace {
     char *meta = nullptr;      // ctrl (select state) | rsum | rpre | digit totals
     char *data = nullptr;      // ping-pong scratch
     size_t data_cap = 0;
+    std::mutex aux_mu;         // held while a call uses `aux` (taken before `mu`)
+    char *aux = nullptr;       // staging of an entry point that then calls a sort or select
+    size_t aux_cap = 0;
 
     uint32_t *rsum() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES); }
     uint32_t *rpre() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + RSUM_BYTES); }
@@ -38,6 +41,10 @@ struct Workspace {
 // `data_bytes` of scratch.  On error returns a status code and sets the
 // thread's error text.
 int acquire_workspace(hipStream_t stream, size_t data_bytes, Workspace **out, std::unique_lock<std::mutex> *lock);
+
+// The aux buffer of (current device, stream), locked, with at least `bytes`.
+// A caller holding it may then acquire the workspace (never the reverse).
+int acquire_aux(hipStream_t stream, size_t bytes, char **out, std::unique_lock<std::mutex> *lock);
 
 int set_error(int code, const char *fmt, ...);
 

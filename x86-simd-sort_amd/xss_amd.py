@@ -121,6 +121,22 @@ def argselect(a: DArray, k, hasnan=False, out: DArray = None):
     return out
 
 
+def argsort_arg(a: DArray, arg: DArray, hasnan=False, descending=False):
+    """NumPy-style argsort (x86simdsortStatic::argsort(T *, size_t *arg, ...)):
+    reorders the u64 indices in `arg` in place so that a[arg] is sorted."""
+    _check(getattr(lib(), f"xss_gpu_argsort_arg_{a.tag}")(a.ptr, arg.ptr, _sz(a.n), int(hasnan), int(descending),
+                                                          _stream()), f"argsort_arg<{a.tag}>")
+    return arg
+
+
+def argselect_arg(a: DArray, arg: DArray, k, hasnan=False):
+    """NumPy-style argselect (x86simdsortStatic::argselect(T *, size_t *arg, k, ...)):
+    reorders `arg` in place so that a[arg[k]] has rank k, smaller keys before it."""
+    _check(getattr(lib(), f"xss_gpu_argselect_arg_{a.tag}")(a.ptr, arg.ptr, _sz(k), _sz(a.n), int(hasnan),
+                                                            _stream()), f"argselect_arg<{a.tag}>")
+    return arg
+
+
 def keyvalue_qsort(key: DArray, val: DArray, hasnan=False, descending=False):
     _check(getattr(lib(), f"xss_gpu_keyvalue_qsort_{key.tag}_{val.tag}")(
         key.ptr, val.ptr, _sz(key.n), int(hasnan), int(descending), _stream()), f"keyvalue_qsort<{key.tag},{val.tag}>")

@@ -13,6 +13,9 @@ c3  keyvalue_qsort u64/u64 (2^27, Zipf s=1.1 keys below 2^32, values = index)
     N-1 and partial_qsort / keyvalue_partial_sort at k = 1000 and N/100 on
     the same arrays; object_qsort of 2^24 {double x, y, z} by the Euclidean
     norm (device key evaluation).
+c6  (with --only c6) the PyTorch shim torch.ops.xss.sort beside torch.sort:
+    2^28 float32 in one row and 4096 rows of 65536, values + int64 indices
+    (bytes per key counted as one read and write of keys and indices).
 Each measurement restores the input from a pristine device copy outside the
 timed region and times the call alone with HIP events on the stream it runs
 on; the median of --reps is reported.  One JSON line per measurement with
@@ -244,6 +247,26 @@ def c3(torch, L, reps):
     emit("c3", "object_qsort", "xyz_f64_norm", n, ms, obj)
 
 
+def c6(torch, L, reps):
+    """The PyTorch shim beside torch.sort on the same GPU: one row of 2^28
+    float32 (random bits, finite) and 4096 rows of 65536 float32."""
+    import xss_torch  # noqa: F401  (registers torch.ops.xss)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(600)
+    for shape in ((1 << 28,), (4096, 1 << 16)):
+        n = 1
+        for d in shape:
+            n *= d
+        x = rand_bits(torch, "f32", n, g).view(torch.float32).view(shape)
+        # each call returns 0 for timed(); its outputs are dropped after the events
+        for name, fn in (("xss::sort", lambda: 0 if torch.ops.xss.sort(x, False) else 1),
+                         ("torch.sort", lambda: 0 if torch.sort(x, dim=-1) else 1)):
+            ms = timed(torch, lambda: None, fn, reps)
+            emit("c6", name, "f32", n, ms, 2 * 4 + 2 * 8, shape="x".join(map(str, shape)))
+        del x
+        torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reps", type=int, default=3)
@@ -259,7 +282,7 @@ def main():
         if c == "c2":
             c2(torch, L, args.reps, types, recipes)
         else:
-            {"c1": c1, "c3": c3}[c](torch, L, args.reps)
+            {"c1": c1, "c3": c3, "c6": c6}[c](torch, L, args.reps)
 
 
 if __name__ == "__main__":

@@ -94,3 +94,30 @@ def test_static_api_resolves_to_library_symbols(tmp_path):
     assert {f"xss_host_argsort_arg_{t}" for t in ("u16", "i16", "f32", "f64", "u32", "i32", "u64", "i64")} <= need
     missing = sorted(need - exported())
     assert not missing, missing
+
+
+def test_cmake_package_links_the_dropin_program(tmp_path):
+    """x86-simd-sort_amd/cmake/xss_amdConfig.cmake (the counterpart of the
+    reference's pkg-config file, meson.build:72-83): a CMake project finds
+    the package, and the C++ drop-in program links against its target (it is
+    not run: no GPU calls here)."""
+    import shutil
+    if shutil.which("cmake") is None:
+        pytest.skip("cmake not installed")
+    proj = tmp_path / "proj"
+    proj.mkdir()
+    (proj / "CMakeLists.txt").write_text(
+        "cmake_minimum_required(VERSION 3.16)\n"
+        "project(use_xss CXX)\n"
+        "find_package(xss_amd 7.0 CONFIG REQUIRED)\n"
+        f"add_executable(dropin {os.path.join(T.REPO, 'tests', 'cpp', 'dropin.cpp')})\n"
+        "target_link_libraries(dropin PRIVATE xss_amd::x86simdsortcpp)\n")
+    build = tmp_path / "build"
+    subprocess.run(["cmake", "-S", str(proj), "-B", str(build), "-DCMAKE_CXX_COMPILER=g++",
+                    f"-Dxss_amd_DIR={os.path.join(T.REPO, 'x86-simd-sort_amd', 'cmake')}"],
+                   check=True, capture_output=True, text=True)
+    subprocess.run(["cmake", "--build", str(build)], check=True, capture_output=True, text=True)
+    exe = build / "dropin"
+    assert exe.exists()
+    needed = subprocess.run(["readelf", "-d", str(exe)], capture_output=True, text=True, check=True).stdout
+    assert "libxss_amd.so" in needed

@@ -554,13 +554,20 @@ struct SegRanges {
 // Tile shape of the segmented passes: 512 threads x 16 items (8 for items
 // wider than 8 bytes), smaller than the one-array pass for 4-byte keys
 // because a segment of a few tiles wastes less of a smaller last tile.  The
-// host-side plan (shape_of below) cuts ranges with the same tile size.
+// host-side plan (shape_of in xss_segmented.hip) cuts ranges with the same
+// tile size.
 // Keys-only 4-byte tiles get a 128-VGPR budget (two workgroups per CU).
+#ifndef XSS_SEG_BLOCK_C1
+#define XSS_SEG_BLOCK_C1 512   // segmented-pass tile of 4-byte items (build-time knobs)
+#endif
+#ifndef XSS_SEG_ITEMS_C1
+#define XSS_SEG_ITEMS_C1 16
+#endif
 template <class KD, int PL>
 struct SegShape {
     static constexpr int BYTES = ScatterClass<KD, PL>::bytes;
-    static constexpr int BLOCK = 512;
-    static constexpr int ITEMS = BYTES > 8 ? 8 : 16;
+    static constexpr int BLOCK = BYTES == 4 ? XSS_SEG_BLOCK_C1 : 512;
+    static constexpr int ITEMS = BYTES > 8 ? 8 : BYTES == 4 ? XSS_SEG_ITEMS_C1 : 16;
     static constexpr int WAVES = BYTES == 4 ? 4 : 1;
 };
 

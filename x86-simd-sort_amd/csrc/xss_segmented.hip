@@ -160,8 +160,8 @@ void shape_of(const std::string &tag, int op, uint32_t *maxn, uint32_t *tile)
     // BlockShape of xss_sort.hpp and SegShape above, for keys-only and index payloads
     const int bpe = 2 * kb + 2;
     *maxn = (65536 / bpe) >= 4096 ? 4096 : (65536 / bpe) >= 2048 ? 2048 : 1024;
-    const int items = (kb + (idx ? 4 : 0)) > 8 ? 8 : 16;
-    *tile = 512u * items;
+    const int bytes = kb + (idx ? 4 : 0);
+    *tile = bytes == 4 ? uint32_t(XSS_SEG_BLOCK_C1) * XSS_SEG_ITEMS_C1 : 512u * (bytes > 8 ? 8 : 16);
 }
 
 }  // namespace

@@ -98,13 +98,52 @@ struct UpArgs {
     const uint64_t *range_begin;  // segmented form: explicit key range of
     const uint64_t *range_end;    // each workgroup (null: blockIdx.x * range_keys)
     const uint32_t *nr_dev;       // device-built range tables: the range count
+    // passes that skip a constant digit: `ran` [pass] = 1 if that pass moved
+    // the keys; with an odd count of moves before this pass they are in keys_alt
+    const void *keys_alt;
+    const uint32_t *ran;
+    uint32_t pass;
 };
+
+// Number of the earlier passes (of a sort that skips constant digits) that
+// moved the keys: its parity says which of the two buffers holds them.  The
+// flags are written by one workgroup of a downsweep while workgroups on
+// other XCDs read the same line, so every access is an agent-scope atomic
+// (coherent across the XCDs' L2 caches).
+__device__ __forceinline__ uint32_t moves_before(const uint32_t *ran, uint32_t pass)
+{
+    uint32_t c = 0;
+    for (uint32_t p = 0; p < pass; ++p)
+        c += __hip_atomic_load(const_cast<uint32_t *>(ran) + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return c;
+}
+// 1 if the keys are in the alternate buffer.  Wave-uniform by construction
+// (readfirstlane): a pointer chosen from a per-lane load would make the
+// compiler treat the pass's loops as divergent, which breaks the ballot-based
+// wave-uniform paths of the histogram and ranking code.
+__device__ __forceinline__ uint32_t moves_odd(const uint32_t *ran, uint32_t pass)
+{
+    return __builtin_amdgcn_readfirstlane(moves_before(ran, pass) & 1u);
+}
 
 template <class KD>
 __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
 {
     using U = typename KD::U;
     __shared__ uint32_t h[256];
+    // the buffer holding the keys: a.keys, or (a sort that skips constant
+    // digits, after an odd number of moves) a.keys_alt.  Chosen by a load
+    // from a 2-entry LDS table rather than a select on the pointers: the
+    // select of two kernel-argument pointers on a readfirstlane'd flag was
+    // compiled to take keys_alt unconditionally (hipcc, ROCm 7.2).
+    __shared__ const void *s_kbuf[2];
+    if (threadIdx.x == 0) {
+        s_kbuf[0] = a.keys;
+        s_kbuf[1] = a.ran ? a.keys_alt : a.keys;
+    }
+    const uint32_t odd = a.ran ? moves_odd(a.ran, a.pass) : 0u;
+    __syncthreads();
+    const U *keys = static_cast<const U *>(s_kbuf[odd]);
     // one range per workgroup, or (device-built tables, launched with a
     // bounded grid) every gridDim.x-th range
     const uint32_t nr = a.nr_dev ? *a.nr_dev : gridDim.x;
@@ -112,7 +151,6 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
         __syncthreads();  // the previous range's histogram was written out
         for (int i = threadIdx.x; i < 256; i += UP_BLOCK) h[i] = 0;
         __syncthreads();
-        const U *keys = static_cast<const U *>(a.keys);
         const U flip = U(a.flip);
         const bool nl = a.nan_last != 0;
         const int shift = int(a.shift);
@@ -247,6 +285,13 @@ struct PassArgs {
     uint32_t ranges;
     const uint32_t *rpre;    // [256][ranges]
     const uint32_t *total;   // [256]
+    // a sort that skips constant digits (keys and values ping-pong between
+    // the *_in and *_out buffers by the parity of the earlier moves, see
+    // UpArgs): `ran` [pass] records whether this pass moved the keys; it
+    // skips when one digit holds all n keys and skip_ok is set
+    uint32_t *ran;
+    uint32_t pass;
+    uint32_t skip_ok;
 };
 
 template <class KD, int PL, int BLOCK, int ITEMS>
@@ -460,15 +505,59 @@ __global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL, BLOCK) void do
     const uint32_t ntiles = (a.n + TILE - 1) / TILE;
     const uint32_t tile0 = blockIdx.x * a.range_tiles;
     const uint32_t tile1 = min(ntiles, tile0 + a.range_tiles);
+    PassArgs b = a;
+    if (a.ran) {
+        // all n keys in one digit: a stable pass leaves them in place
+        const bool constant = __syncthreads_or(tid < 256 && a.total[tid] == a.n) != 0;
+        const bool skip = constant && a.skip_ok;
+        if (blockIdx.x == 0 && tid == 0)
+            __hip_atomic_store(a.ran + a.pass, skip ? 0u : 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (skip) return;
+        if (moves_odd(a.ran, a.pass)) {  // the keys are in the *_out buffers
+            b.keys_in = a.keys_out;
+            b.keys_out = const_cast<void *>(a.keys_in);
+            b.vals_in = a.vals_out;
+            b.vals_out = const_cast<void *>(a.vals_in);
+        }
+    }
     // running global position of each digit of this range (thread d owns digit d)
-    const uint32_t dstart = digit_starts(a.total, sm.wsum);
-    uint64_t run = tid < 256 ? dstart + a.rpre[size_t(tid) * a.ranges + blockIdx.x] : 0u;
-    const uint32_t nfull = min(tile1, a.n / TILE);
+    const uint32_t dstart = digit_starts(b.total, sm.wsum);
+    uint64_t run = tid < 256 ? dstart + b.rpre[size_t(tid) * b.ranges + blockIdx.x] : 0u;
+    const uint32_t nfull = min(tile1, b.n / TILE);
     uint32_t tile = tile0;
     for (; tile < nfull; ++tile)
-        downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(a, sm, uint64_t(tile) * TILE, TILE, 0, run);
+        downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(b, sm, uint64_t(tile) * TILE, TILE, 0, run);
     for (; tile < tile1; ++tile)
-        downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, uint64_t(tile) * TILE, a.n - tile * TILE, 0, run);
+        downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(b, sm, uint64_t(tile) * TILE, b.n - tile * TILE, 0, run);
+}
+
+// After a sort that skips constant digits: an odd number of moves left the
+// keys (and values) in the scratch buffers; copy them back.
+template <int = 0>
+__global__ __launch_bounds__(256) void skip_finish_kernel(void *keys, const void *keys_tmp, void *vals,
+                                                          const void *vals_tmp, uint64_t key_bytes,
+                                                          uint64_t val_bytes, const uint32_t *ran, uint32_t passes)
+{
+    if (!moves_odd(ran, passes)) return;
+    const uint64_t stride = uint64_t(gridDim.x) * 256u;
+    auto copy = [&](void *dst, const void *src, uint64_t bytes) {
+        // both buffers are 256-byte aligned allocations or the caller's array:
+        // 16-byte vectors where both ends allow it, bytes otherwise
+        const bool vec = ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15u) == 0;
+        if (vec) {
+            const uint64_t nv = bytes / 16;
+            uint4 *d = static_cast<uint4 *>(dst);
+            const uint4 *q = static_cast<const uint4 *>(src);
+            for (uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x; i < nv; i += stride) d[i] = q[i];
+            for (uint64_t i = nv * 16 + uint64_t(blockIdx.x) * 256u + threadIdx.x; i < bytes; i += stride)
+                static_cast<uint8_t *>(dst)[i] = static_cast<const uint8_t *>(src)[i];
+        } else {
+            for (uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x; i < bytes; i += stride)
+                static_cast<uint8_t *>(dst)[i] = static_cast<const uint8_t *>(src)[i];
+        }
+    };
+    copy(keys, keys_tmp, key_bytes);
+    if (vals) copy(vals, vals_tmp, val_bytes);
 }
 
 // ---------------------------------------------------------------- segmented passes

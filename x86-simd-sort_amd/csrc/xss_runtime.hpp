@@ -19,13 +19,13 @@ namespace xss {
 
 constexpr size_t CTRL_BYTES = 256;
 constexpr size_t RSUM_BYTES = size_t(MAX_RANGES) * 256 * sizeof(uint32_t);
-constexpr size_t META_BYTES = CTRL_BYTES + 2 * RSUM_BYTES + 1024;
+constexpr size_t META_BYTES = CTRL_BYTES + 2 * RSUM_BYTES + 1024 + 256;
 
 struct Workspace {
     int device = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;             // held while a call enqueues work on this workspace
-    char *meta = nullptr;      // ctrl (select state) | rsum | rpre | digit totals
+    char *meta = nullptr;      // ctrl (select state) | rsum | rpre | digit totals | pass moves
     char *data = nullptr;      // ping-pong scratch
     size_t data_cap = 0;
     std::mutex aux_mu;         // held while a call uses `aux` (taken before `mu`)
@@ -35,6 +35,8 @@ struct Workspace {
     uint32_t *rsum() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES); }
     uint32_t *rpre() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + RSUM_BYTES); }
     uint32_t *total() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + 2 * RSUM_BYTES); }
+    // per pass of an LSD sort that skips constant digits: 1 if it moved the keys
+    uint32_t *ran() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + 2 * RSUM_BYTES + 1024); }
 };
 
 // Returns the workspace of (current device, stream), locked, with at least

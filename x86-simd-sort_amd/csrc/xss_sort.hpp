@@ -115,10 +115,12 @@ int launch_downsweep(const PassArgs &a, uint32_t ranges, bool nan_last, hipStrea
     return XSS_OK;
 }
 
+// ran != null: a pass of a sort that skips constant digits (src/dst are
+// then the two buffers the keys ping-pong between, chosen on the device).
 template <class KD, int PL>
 int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *dst_k, void *dst_v,
                      uint64_t *idx_out, size_t n, int shift, uint64_t flip, bool nan_last, bool first, bool last,
-                     bool canon_nan, hipStream_t s)
+                     bool canon_nan, hipStream_t s, uint32_t *ran = nullptr, uint32_t pass = 0, bool skip_ok = false)
 {
     using Plan = SortPlan<KD, PL>;
     const uint32_t nn = uint32_t(n);
@@ -126,6 +128,7 @@ int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *
     UpArgs u{};
     u.keys = src_k; u.n = nn; u.range_keys = uint32_t(rtiles * Plan::tile(n)); u.shift = uint32_t(shift);
     u.nan_last = nan_last; u.flip = flip; u.rsum = ws->rsum(); u.ranges = ranges;
+    u.keys_alt = dst_k; u.ran = ran; u.pass = pass;
     {
         KernelTimer t("upsweep", s);
         hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
@@ -143,6 +146,7 @@ int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *
     a.n = nn; a.shift = uint32_t(shift); a.flip = flip; a.nan_last = nan_last; a.range_tiles = rtiles;
     a.ranges = ranges; a.rpre = ws->rpre(); a.total = ws->total();
     a.first = first; a.last = last; a.canon_nan = canon_nan;
+    a.ran = ran; a.pass = pass; a.skip_ok = skip_ok;
     if (Plan::small_tiles(n)) return launch_downsweep<KD, PL, SmallPassShape<KD, PL>>(a, ranges, nan_last, s);
     return launch_downsweep<KD, PL, PassShape<KD, PL>>(a, ranges, nan_last, s);
 }
@@ -191,6 +195,28 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
         }
     } else if (PL != PL_NONE) {
         v0 = k0 + align256(n * sizeof(U));
+    }
+    if (PL != PL_IDX && keys_in == keys_out && vals_in == vals_out) {
+        // In place: a pass whose digit is the same for all n keys (keys of a
+        // narrow range, constant arrays) is skipped on the device; the keys
+        // ping-pong between the array and the scratch buffer by the parity of
+        // the passes that ran.  The last pass of a NaN-canonicalising qsort
+        // always runs (it writes the canonical NaN).
+        uint32_t *ran = ws->ran();
+        for (int p = 0; p < P; ++p) {
+            const bool last = p == P - 1;
+            const bool canon_pass = last && canon && nan_last;
+            const int rc = enqueue_lsd_pass<KD, PL>(ws, keys_out, vals_out, k0, v0, idx_out, n, 8 * p, flip, nan_last,
+                                                    p == 0, last, canon_pass, s, ran, uint32_t(p), !canon_pass);
+            if (rc) return rc;
+        }
+        using V = typename PayloadT<PL>::type;
+        const uint64_t kb = uint64_t(n) * sizeof(U), vb = PL == PL_NONE ? 0 : uint64_t(n) * sizeof(V);
+        const uint32_t blocks = uint32_t(std::min<uint64_t>(4096, (kb / 16 + 255) / 256 + 1));
+        hipLaunchKernelGGL(skip_finish_kernel<0>, dim3(blocks), dim3(256), 0, s, keys_out, k0,
+                           PL == PL_NONE ? nullptr : vals_out, v0, kb, vb, ran, uint32_t(P));
+        XSS_HIP_CHECK(hipGetLastError());
+        return XSS_OK;
     }
     const void *src_k = keys_in, *src_v = vals_in;
     for (int p = 0; p < P; ++p) {

@@ -23,6 +23,7 @@
 // step is stable anyway.
 #pragma once
 #include <atomic>
+#include <type_traits>
 #include <vector>
 
 #include "xss_count.hpp"
@@ -192,20 +193,29 @@ __global__ __launch_bounds__(1024) void hyb_large_ranges_kernel(HybArgs a)
 // Shape of the per-bucket LDS sort: 512 threads, up to ITEMS keys (and
 // payloads) each.  4-byte keys only: compiled for 8 waves per SIMD (its
 // SGPRs alone would cap it at 7).
-template <class KD, int PL>
+#ifndef XSS_HYB_ITEMS_W8
+#define XSS_HYB_ITEMS_W8 12  // items per thread for 5..8-byte items (build-time knob)
+#endif
+// SMALL (5..8-byte items only): 6 items per thread, a 3072-key bucket
+// capacity for arrays whose buckets average at most HYB_SMALL_AVG keys; the
+// smaller LDS footprint fits more workgroups per CU (64-bit keys at 2^27:
+// 3.52 vs 4.21 ms per sort).
+template <class KD, int PL, bool SMALL = false>
 struct HybShape {
     static constexpr int BYTES = ScatterClass<KD, PL>::bytes;
-    static constexpr int ITEMS = BYTES > 8 ? 8 : HYB_ITEMS;
+    static constexpr int ITEMS = SMALL ? 6 : BYTES > 8 ? 8 : BYTES > 4 ? XSS_HYB_ITEMS_W8 : HYB_ITEMS;
     static constexpr uint32_t CAP = uint32_t(HYB_BLOCK) * ITEMS;
     static constexpr int WAVES = BYTES == 4 ? XSS_HYB_WAVES : 1;
+    static constexpr bool HAS_SMALL = BYTES > 4 && BYTES <= 8;
 };
+constexpr uint32_t HYB_SMALL_AVG = 2304;  // largest average bucket for the SMALL shape
 
 // One workgroup per bucket of at most CAP keys: LSD radix by the low P - 2
 // digits in LDS (the top two are the bucket's), written back in place; the
 // payload (values, or argsort's 32-bit indices) moves with its key.
 // PL_IDX: the sorted indices go to idx_out (64-bit) at the bucket's place.
-template <class KD, int PL, bool NL>
-__global__ __launch_bounds__(HYB_BLOCK) __attribute__((amdgpu_waves_per_eu(HybShape<KD, PL>::WAVES)))
+template <class KD, int PL, bool NL, bool SMALL>
+__global__ __launch_bounds__(HYB_BLOCK) __attribute__((amdgpu_waves_per_eu(HybShape<KD, PL, SMALL>::WAVES)))
 void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_v, void *vals_v, uint64_t *idx_out,
                       uint64_t flip64, uint32_t canon)
 {
@@ -214,8 +224,8 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
     constexpr bool HAS_PL = PL != PL_NONE;
     constexpr int P = KD::PASSES;
     constexpr int WAVES = HYB_BLOCK / 64;
-    constexpr int ITEMS = HybShape<KD, PL>::ITEMS;
-    constexpr uint32_t CAP = HybShape<KD, PL>::CAP;
+    constexpr int ITEMS = HybShape<KD, PL, SMALL>::ITEMS;
+    constexpr uint32_t CAP = HybShape<KD, PL, SMALL>::CAP;
     __shared__ U s_keys[CAP];
     __shared__ V s_vals[HAS_PL ? CAP : 1];
     __shared__ uint32_t s_off[WAVES * 256];
@@ -347,6 +357,10 @@ struct HybPlan {
     using V = typename PayloadT<PL>::type;
     static constexpr bool HAS_PL = PL != PL_NONE;
     static constexpr uint32_t CAP = HybShape<KD, PL>::CAP;
+    // the bucket capacity of the local sort for n keys (SMALL shape when the
+    // buckets average at most HYB_SMALL_AVG keys)
+    static bool small(size_t n) { return HybShape<KD, PL>::HAS_SMALL && n / HYB_BUCKETS <= HYB_SMALL_AVG; }
+    static uint32_t cap(size_t n) { return small(n) ? HybShape<KD, PL, true>::CAP : CAP; }
     // tile shape of the segmented passes (8K-key tiles for 4-byte keys
     // measured faster here than the one-array pass's 16K)
     using SS = SegShape<KD, PL>;
@@ -359,7 +373,7 @@ struct HybPlan {
     }
     static uint32_t cap_b(size_t n) { return uint32_t((n + rk_b(n) - 1) / rk_b(n) + 256); }
     static uint64_t rk_d(size_t) { return 8 * SEG_TILE; }
-    static uint32_t lcap(size_t n) { return uint32_t(std::min<size_t>(HYB_BUCKETS, n / (CAP + 1) + 1)); }
+    static uint32_t lcap(size_t n) { return uint32_t(std::min<size_t>(HYB_BUCKETS, n / (cap(n) + 1) + 1)); }
     static uint32_t cap_d(size_t n) { return uint32_t((n + rk_d(n) - 1) / rk_d(n) + lcap(n)); }
     static size_t table_bytes(uint32_t cap) { return 3 * align256(size_t(cap) * 8) + 2 * align256(size_t(cap) * 4); }
     // key / payload buffers: keys-only and key-value sorts ping-pong with one
@@ -437,7 +451,7 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     h.segtot = segtot;
     h.nlarge = ctr;
     h.lcap = lcap;
-    h.cap = HP::CAP;
+    h.cap = HP::cap(n);
     h.rk_b = HP::rk_b(n);
     h.rk_d = HP::rk_d(n);
 
@@ -500,12 +514,21 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     XSS_HIP_CHECK(hipGetLastError());
     {
         KernelTimer t("hyb_local", s);
-        if (nan_last)
-            hipLaunchKernelGGL((hyb_local_kernel<KD, PL, true>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart,
-                               h.bsize, xk, xv, idx_out, flip, uint32_t(canon_nan));
-        else
-            hipLaunchKernelGGL((hyb_local_kernel<KD, PL, false>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s, h.bstart,
-                               h.bsize, xk, xv, idx_out, flip, 0u);
+        auto launch = [&](auto small_tag) {
+            constexpr bool SM = decltype(small_tag)::value;
+            if (nan_last)
+                hipLaunchKernelGGL((hyb_local_kernel<KD, PL, true, SM>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s,
+                                   h.bstart, h.bsize, xk, xv, idx_out, flip, uint32_t(canon_nan));
+            else
+                hipLaunchKernelGGL((hyb_local_kernel<KD, PL, false, SM>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s,
+                                   h.bstart, h.bsize, xk, xv, idx_out, flip, 0u);
+        };
+        if constexpr (HybShape<KD, PL>::HAS_SMALL) {
+            if (HP::small(n)) launch(std::true_type{});
+            else launch(std::false_type{});
+        } else {
+            launch(std::false_type{});
+        }
     }
     XSS_HIP_CHECK(hipGetLastError());
     {

@@ -196,19 +196,22 @@ __global__ __launch_bounds__(1024) void hyb_large_ranges_kernel(HybArgs a)
 #ifndef XSS_HYB_ITEMS_W8
 #define XSS_HYB_ITEMS_W8 12  // items per thread for 5..8-byte items (build-time knob)
 #endif
-// SMALL (5..8-byte items only): 6 items per thread, a 3072-key bucket
-// capacity for arrays whose buckets average at most HYB_SMALL_AVG keys; the
+// SMALL (items wider than 4 bytes): 6 items per thread (5 for items wider
+// than 8 bytes), a 3072- (2560-) key bucket capacity, for arrays whose
+// buckets average at most that capacity less HYB_SMALL_MARGIN keys; the
 // smaller LDS footprint fits more workgroups per CU (64-bit keys at 2^27:
 // 3.52 vs 4.21 ms per sort).
 template <class KD, int PL, bool SMALL = false>
 struct HybShape {
     static constexpr int BYTES = ScatterClass<KD, PL>::bytes;
-    static constexpr int ITEMS = SMALL ? 6 : BYTES > 8 ? 8 : BYTES > 4 ? XSS_HYB_ITEMS_W8 : HYB_ITEMS;
+    static constexpr int ITEMS = SMALL ? (BYTES > 8 ? 5 : 6) : BYTES > 8 ? 8 : BYTES > 4 ? XSS_HYB_ITEMS_W8 : HYB_ITEMS;
     static constexpr uint32_t CAP = uint32_t(HYB_BLOCK) * ITEMS;
     static constexpr int WAVES = BYTES == 4 ? XSS_HYB_WAVES : 1;
-    static constexpr bool HAS_SMALL = BYTES > 4 && BYTES <= 8;
+    static constexpr bool HAS_SMALL = BYTES > 4;
 };
-constexpr uint32_t HYB_SMALL_AVG = 2304;  // largest average bucket for the SMALL shape
+// average bucket size below the SMALL capacity (about 8 standard deviations
+// of a uniform bucket's size at ~2000 keys)
+constexpr uint32_t HYB_SMALL_MARGIN = 384;
 
 // One workgroup per bucket of at most CAP keys: LSD radix by the low P - 2
 // digits in LDS (the top two are the bucket's), written back in place; the
@@ -358,8 +361,11 @@ struct HybPlan {
     static constexpr bool HAS_PL = PL != PL_NONE;
     static constexpr uint32_t CAP = HybShape<KD, PL>::CAP;
     // the bucket capacity of the local sort for n keys (SMALL shape when the
-    // buckets average at most HYB_SMALL_AVG keys)
-    static bool small(size_t n) { return HybShape<KD, PL>::HAS_SMALL && n / HYB_BUCKETS <= HYB_SMALL_AVG; }
+    // buckets average at most its capacity less HYB_SMALL_MARGIN keys)
+    static bool small(size_t n)
+    {
+        return HybShape<KD, PL>::HAS_SMALL && n / HYB_BUCKETS + HYB_SMALL_MARGIN <= HybShape<KD, PL, true>::CAP;
+    }
     static uint32_t cap(size_t n) { return small(n) ? HybShape<KD, PL, true>::CAP : CAP; }
     // tile shape of the segmented passes (8K-key tiles for 4-byte keys
     // measured faster here than the one-array pass's 16K)

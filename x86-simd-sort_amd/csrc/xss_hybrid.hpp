@@ -201,13 +201,22 @@ __global__ __launch_bounds__(1024) void hyb_large_ranges_kernel(HybArgs a)
 // buckets average at most that capacity less HYB_SMALL_MARGIN keys; the
 // smaller LDS footprint fits more workgroups per CU (64-bit keys at 2^27:
 // 3.52 vs 4.21 ms per sort).
-template <class KD, int PL, bool SMALL = false>
+// MID (items of at most 8 bytes): 9 items per thread, a 4608-key capacity
+// for buckets averaging at most that less HYB_SMALL_MARGIN (argsort of
+// 32-bit keys at 2^28: 4.61 vs 4.83 ms).  SIZE: 0 = the default shape,
+// 1 = MID, 2 = SMALL.
+template <class KD, int PL, int SIZE = 0>
 struct HybShape {
     static constexpr int BYTES = ScatterClass<KD, PL>::bytes;
-    static constexpr int ITEMS = SMALL ? (BYTES > 8 ? 5 : 6) : BYTES > 8 ? 8 : BYTES > 4 ? XSS_HYB_ITEMS_W8 : HYB_ITEMS;
+    static constexpr int ITEMS = SIZE == 2   ? (BYTES > 8 ? 5 : 6)
+                               : SIZE == 1   ? 9
+                               : BYTES > 8   ? 8
+                               : BYTES > 4   ? XSS_HYB_ITEMS_W8
+                                             : HYB_ITEMS;
     static constexpr uint32_t CAP = uint32_t(HYB_BLOCK) * ITEMS;
     static constexpr int WAVES = BYTES == 4 ? XSS_HYB_WAVES : 1;
     static constexpr bool HAS_SMALL = BYTES > 4;
+    static constexpr bool HAS_MID = BYTES <= 8;
 };
 // average bucket size below the SMALL capacity (about 8 standard deviations
 // of a uniform bucket's size at ~2000 keys)
@@ -217,8 +226,8 @@ constexpr uint32_t HYB_SMALL_MARGIN = 384;
 // digits in LDS (the top two are the bucket's), written back in place; the
 // payload (values, or argsort's 32-bit indices) moves with its key.
 // PL_IDX: the sorted indices go to idx_out (64-bit) at the bucket's place.
-template <class KD, int PL, bool NL, bool SMALL>
-__global__ __launch_bounds__(HYB_BLOCK) __attribute__((amdgpu_waves_per_eu(HybShape<KD, PL, SMALL>::WAVES)))
+template <class KD, int PL, bool NL, int SIZE>
+__global__ __launch_bounds__(HYB_BLOCK) __attribute__((amdgpu_waves_per_eu(HybShape<KD, PL, SIZE>::WAVES)))
 void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_v, void *vals_v, uint64_t *idx_out,
                       uint64_t flip64, uint32_t canon)
 {
@@ -227,8 +236,8 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
     constexpr bool HAS_PL = PL != PL_NONE;
     constexpr int P = KD::PASSES;
     constexpr int WAVES = HYB_BLOCK / 64;
-    constexpr int ITEMS = HybShape<KD, PL, SMALL>::ITEMS;
-    constexpr uint32_t CAP = HybShape<KD, PL, SMALL>::CAP;
+    constexpr int ITEMS = HybShape<KD, PL, SIZE>::ITEMS;
+    constexpr uint32_t CAP = HybShape<KD, PL, SIZE>::CAP;
     __shared__ U s_keys[CAP];
     __shared__ V s_vals[HAS_PL ? CAP : 1];
     __shared__ uint32_t s_off[WAVES * 256];
@@ -360,13 +369,20 @@ struct HybPlan {
     using V = typename PayloadT<PL>::type;
     static constexpr bool HAS_PL = PL != PL_NONE;
     static constexpr uint32_t CAP = HybShape<KD, PL>::CAP;
-    // the bucket capacity of the local sort for n keys (SMALL shape when the
-    // buckets average at most its capacity less HYB_SMALL_MARGIN keys)
-    static bool small(size_t n)
+    // the shape (and bucket capacity) of the local sort for n keys: the
+    // smallest whose capacity exceeds the average bucket by HYB_SMALL_MARGIN
+    static int shape(size_t n)
     {
-        return HybShape<KD, PL>::HAS_SMALL && n / HYB_BUCKETS + HYB_SMALL_MARGIN <= HybShape<KD, PL, true>::CAP;
+        const size_t need = n / HYB_BUCKETS + HYB_SMALL_MARGIN;
+        if (HybShape<KD, PL>::HAS_SMALL && need <= HybShape<KD, PL, 2>::CAP) return 2;
+        if (HybShape<KD, PL>::HAS_MID && need <= HybShape<KD, PL, 1>::CAP) return 1;
+        return 0;
     }
-    static uint32_t cap(size_t n) { return small(n) ? HybShape<KD, PL, true>::CAP : CAP; }
+    static uint32_t cap(size_t n)
+    {
+        const int sh = shape(n);
+        return sh == 2 ? HybShape<KD, PL, 2>::CAP : sh == 1 ? HybShape<KD, PL, 1>::CAP : CAP;
+    }
     // tile shape of the segmented passes (8K-key tiles for 4-byte keys
     // measured faster here than the one-array pass's 16K)
     using SS = SegShape<KD, PL>;
@@ -520,21 +536,23 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     XSS_HIP_CHECK(hipGetLastError());
     {
         KernelTimer t("hyb_local", s);
-        auto launch = [&](auto small_tag) {
-            constexpr bool SM = decltype(small_tag)::value;
+        auto launch = [&](auto size_tag) {
+            constexpr int SZ = decltype(size_tag)::value;
             if (nan_last)
-                hipLaunchKernelGGL((hyb_local_kernel<KD, PL, true, SM>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s,
+                hipLaunchKernelGGL((hyb_local_kernel<KD, PL, true, SZ>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s,
                                    h.bstart, h.bsize, xk, xv, idx_out, flip, uint32_t(canon_nan));
             else
-                hipLaunchKernelGGL((hyb_local_kernel<KD, PL, false, SM>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s,
+                hipLaunchKernelGGL((hyb_local_kernel<KD, PL, false, SZ>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s,
                                    h.bstart, h.bsize, xk, xv, idx_out, flip, 0u);
         };
+        const int sh = HP::shape(n);
         if constexpr (HybShape<KD, PL>::HAS_SMALL) {
-            if (HP::small(n)) launch(std::true_type{});
-            else launch(std::false_type{});
-        } else {
-            launch(std::false_type{});
+            if (sh == 2) launch(std::integral_constant<int, 2>{});
         }
+        if constexpr (HybShape<KD, PL>::HAS_MID) {
+            if (sh == 1) launch(std::integral_constant<int, 1>{});
+        }
+        if (sh == 0) launch(std::integral_constant<int, 0>{});
     }
     XSS_HIP_CHECK(hipGetLastError());
     {

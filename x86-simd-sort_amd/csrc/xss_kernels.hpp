@@ -103,6 +103,11 @@ struct UpArgs {
     const void *keys_alt;
     const uint32_t *ran;
     uint32_t pass;
+    // andor (with ran): AND and OR of the order keys, accumulated by pass 0's
+    // upsweep; a later pass whose digit they show constant skips its read
+    // (skip_ok: the pass may skip)
+    uint64_t *andor;
+    uint32_t skip_ok;
 };
 
 // Number of the earlier passes (of a sort that skips constant digits) that
@@ -116,6 +121,14 @@ __device__ __forceinline__ uint32_t moves_before(const uint32_t *ran, uint32_t p
     for (uint32_t p = 0; p < pass; ++p)
         c += __hip_atomic_load(const_cast<uint32_t *>(ran) + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return c;
+}
+// The digit at `shift` is the same in every key: AND and OR of the order
+// keys agree on it (andor written by pass 0's upsweep; wave-uniform).
+__device__ __forceinline__ bool digit_constant(const uint64_t *andor, int shift)
+{
+    const uint64_t x = __hip_atomic_load(const_cast<uint64_t *>(andor), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ^
+                       __hip_atomic_load(const_cast<uint64_t *>(andor) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __builtin_amdgcn_readfirstlane(uint32_t((x >> shift) & 0xffu) == 0u ? 1u : 0u) != 0u;
 }
 // 1 if the keys are in the alternate buffer.  Wave-uniform by construction
 // (readfirstlane): a pointer chosen from a per-lane load would make the
@@ -137,13 +150,20 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
     // select of two kernel-argument pointers on a readfirstlane'd flag was
     // compiled to take keys_alt unconditionally (hipcc, ROCm 7.2).
     __shared__ const void *s_kbuf[2];
+    __shared__ uint64_t s_andor[2];
+    // a later pass over a digit every key shares reads nothing
+    if (a.andor && a.pass > 0 && a.skip_ok && digit_constant(a.andor, int(a.shift))) return;
+    const bool track = a.andor && a.pass == 0;  // pass 0: AND / OR of all order keys
     if (threadIdx.x == 0) {
         s_kbuf[0] = a.keys;
         s_kbuf[1] = a.ran ? a.keys_alt : a.keys;
+        s_andor[0] = ~0ull;
+        s_andor[1] = 0ull;
     }
     const uint32_t odd = a.ran ? moves_odd(a.ran, a.pass) : 0u;
     __syncthreads();
     const U *keys = static_cast<const U *>(s_kbuf[odd]);
+    U t_and = U(~U(0)), t_or = U(0);
     // one range per workgroup, or (device-built tables, launched with a
     // bounded grid) every gridDim.x-th range
     const uint32_t nr = a.nr_dev ? *a.nr_dev : gridDim.x;
@@ -169,6 +189,11 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
             const U *e = reinterpret_cast<const U *>(&q);
 #pragma unroll
             for (int j = 0; j < VEC; ++j) {
+                if (track) {
+                    const U o = KD::ord(e[j], flip, nl);
+                    t_and &= o;
+                    t_or |= o;
+                }
                 const uint32_t d = KD::digit(e[j], flip, nl, shift);
                 const uint64_t act = __ballot(1);
                 const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
@@ -194,11 +219,33 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
         const uint32_t rest = head + (r1 - t0);
         for (uint32_t r = threadIdx.x; r < rest; r += UP_BLOCK) {
             const uint32_t i = r < head ? r0 + r : t0 + (r - head);
+            if (track) {
+                const U o = KD::ord(keys[i], flip, nl);
+                t_and &= o;
+                t_or |= o;
+            }
             atomicAdd(&h[KD::digit(keys[i], flip, nl, shift)], 1u);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < 256; i += UP_BLOCK) a.rsum[size_t(i) * a.ranges + rg] = h[i];
     }
+    if (track) {
+        atomicAnd(reinterpret_cast<unsigned long long *>(&s_andor[0]), (unsigned long long)t_and | ~(unsigned long long)U(~U(0)));
+        atomicOr(reinterpret_cast<unsigned long long *>(&s_andor[1]), (unsigned long long)t_or);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAnd(reinterpret_cast<unsigned long long *>(a.andor), (unsigned long long)s_andor[0]);
+            atomicOr(reinterpret_cast<unsigned long long *>(a.andor + 1), (unsigned long long)s_andor[1]);
+        }
+    }
+}
+
+// AND = all ones, OR = 0 before a sort that skips constant digits
+template <int = 0>
+__global__ void andor_init_kernel(uint64_t *andor)
+{
+    andor[0] = ~0ull;
+    andor[1] = 0ull;
 }
 
 // ---------------------------------------------------------------- range scan
@@ -292,6 +339,7 @@ struct PassArgs {
     uint32_t *ran;
     uint32_t pass;
     uint32_t skip_ok;
+    const uint64_t *andor;  // AND / OR of the order keys (pass 0's upsweep), or null
 };
 
 template <class KD, int PL, int BLOCK, int ITEMS>
@@ -508,7 +556,8 @@ __global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL, BLOCK) void do
     PassArgs b = a;
     if (a.ran) {
         // all n keys in one digit: a stable pass leaves them in place
-        const bool constant = __syncthreads_or(tid < 256 && a.total[tid] == a.n) != 0;
+        const bool constant = a.andor ? digit_constant(a.andor, int(a.shift))
+                                      : __syncthreads_or(tid < 256 && a.total[tid] == a.n) != 0;
         const bool skip = constant && a.skip_ok;
         if (blockIdx.x == 0 && tid == 0)
             __hip_atomic_store(a.ran + a.pass, skip ? 0u : 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -37,6 +37,8 @@ struct Workspace {
     uint32_t *total() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + 2 * RSUM_BYTES); }
     // per pass of an LSD sort that skips constant digits: 1 if it moved the keys
     uint32_t *ran() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + 2 * RSUM_BYTES + 1024); }
+    // AND / OR of the order keys of such a sort (after the 8 pass flags)
+    uint64_t *andor() const { return reinterpret_cast<uint64_t *>(meta + CTRL_BYTES + 2 * RSUM_BYTES + 1024 + 64); }
 };
 
 // Returns the workspace of (current device, stream), locked, with at least

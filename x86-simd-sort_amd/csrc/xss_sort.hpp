@@ -120,7 +120,8 @@ int launch_downsweep(const PassArgs &a, uint32_t ranges, bool nan_last, hipStrea
 template <class KD, int PL>
 int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *dst_k, void *dst_v,
                      uint64_t *idx_out, size_t n, int shift, uint64_t flip, bool nan_last, bool first, bool last,
-                     bool canon_nan, hipStream_t s, uint32_t *ran = nullptr, uint32_t pass = 0, bool skip_ok = false)
+                     bool canon_nan, hipStream_t s, uint32_t *ran = nullptr, uint32_t pass = 0, bool skip_ok = false,
+                     uint64_t *andor = nullptr)
 {
     using Plan = SortPlan<KD, PL>;
     const uint32_t nn = uint32_t(n);
@@ -128,7 +129,7 @@ int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *
     UpArgs u{};
     u.keys = src_k; u.n = nn; u.range_keys = uint32_t(rtiles * Plan::tile(n)); u.shift = uint32_t(shift);
     u.nan_last = nan_last; u.flip = flip; u.rsum = ws->rsum(); u.ranges = ranges;
-    u.keys_alt = dst_k; u.ran = ran; u.pass = pass;
+    u.keys_alt = dst_k; u.ran = ran; u.pass = pass; u.andor = andor; u.skip_ok = skip_ok;
     {
         KernelTimer t("upsweep", s);
         hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
@@ -146,7 +147,7 @@ int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *
     a.n = nn; a.shift = uint32_t(shift); a.flip = flip; a.nan_last = nan_last; a.range_tiles = rtiles;
     a.ranges = ranges; a.rpre = ws->rpre(); a.total = ws->total();
     a.first = first; a.last = last; a.canon_nan = canon_nan;
-    a.ran = ran; a.pass = pass; a.skip_ok = skip_ok;
+    a.ran = ran; a.pass = pass; a.skip_ok = skip_ok; a.andor = andor;
     if (Plan::small_tiles(n)) return launch_downsweep<KD, PL, SmallPassShape<KD, PL>>(a, ranges, nan_last, s);
     return launch_downsweep<KD, PL, PassShape<KD, PL>>(a, ranges, nan_last, s);
 }
@@ -203,11 +204,14 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
         // the passes that ran.  The last pass of a NaN-canonicalising qsort
         // always runs (it writes the canonical NaN).
         uint32_t *ran = ws->ran();
+        uint64_t *andor = ws->andor();
+        hipLaunchKernelGGL(andor_init_kernel<0>, dim3(1), dim3(1), 0, s, andor);
+        XSS_HIP_CHECK(hipGetLastError());
         for (int p = 0; p < P; ++p) {
             const bool last = p == P - 1;
             const bool canon_pass = last && canon && nan_last;
             const int rc = enqueue_lsd_pass<KD, PL>(ws, keys_out, vals_out, k0, v0, idx_out, n, 8 * p, flip, nan_last,
-                                                    p == 0, last, canon_pass, s, ran, uint32_t(p), !canon_pass);
+                                                    p == 0, last, canon_pass, s, ran, uint32_t(p), !canon_pass, andor);
             if (rc) return rc;
         }
         using V = typename PayloadT<PL>::type;

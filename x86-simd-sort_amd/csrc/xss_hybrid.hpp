@@ -280,8 +280,9 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
         key[i] = ok ? keys[idx] : U(0);
         if constexpr (HAS_PL) val[i] = ok ? vals[idx] : V(0);
     }
-    // one stable LSD pass over the digit at `shift`
-    auto pass = [&](const int shift) {
+    // one LSD pass over the digit at `shift`; stable unless `unst` (keys
+    // only, first pass: its input order does not matter)
+    auto pass = [&](const int shift, const bool unst) {
         for (uint32_t i = tid; i < WAVES * 256; i += HYB_BLOCK) s_off[i] = 0;
         __syncthreads();  // counters cleared; the previous pass's reads of s_keys are done
         // rank: stable within the bucket (wave-striped order), per-wave
@@ -296,6 +297,19 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
             if (uint32_t(i) < it) {
                 const uint32_t idx = lbase + i * 64;
                 const uint32_t d = idx < m ? KD::digit(key[i], flip, NL, shift) : 255u;
+                if (PL == PL_NONE && unst) {
+                    // each key reserves its own slot with one returning LDS
+                    // add (a wave of one digit: 64 slots by lane 0, bit 24);
+                    // padding takes no slot (bit 25), so the keys fill
+                    // [0, m) and the next pass sees padding at idx >= m
+                    const bool real = idx < m;
+                    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+                    const bool uni = __ballot(!real || d != d0) == 0;
+                    uint32_t old = 0;
+                    if (real && (!uni || lane == 0)) old = atomicAdd(&wc[d], uni ? 64u : 1u);
+                    rd[i] = old | (d << 16) | (uni ? (1u << 24) : 0u) | (real ? 0u : (1u << 25));
+                    continue;
+                }
                 const uint64_t peers = match_any8(d);
                 const uint32_t below = lanes_below(peers);
                 uint32_t old = 0;
@@ -333,8 +347,13 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
             if (uint32_t(i) < it) {
-                const uint32_t d = rd[i] >> 16;
-                const uint32_t slot = wc[d] + (rd[i] & 0xffffu);
+                uint32_t r = rd[i];
+                if (PL == PL_NONE && unst) {
+                    if (r >> 25) continue;
+                    if (r >> 24) r = (__builtin_amdgcn_readfirstlane(r) + lane) & 0xffffffu;
+                }
+                const uint32_t d = r >> 16;
+                const uint32_t slot = wc[d] + (r & 0xffffu);
                 s_keys[slot] = key[i];
                 if constexpr (HAS_PL) s_vals[slot] = val[i];
             }
@@ -350,11 +369,11 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
     };
     if constexpr (P == 4) {
         // 4-byte keys: both passes inline (measured 8% faster than the loop)
-        pass(0);
-        pass(8);
+        pass(0, true);
+        pass(8, false);
     } else {
 #pragma unroll 1
-        for (int p = 0; p < P - 2; ++p) pass(8 * p);
+        for (int p = 0; p < P - 2; ++p) pass(8 * p, p == 0);
     }
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
@@ -480,7 +499,7 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     // pass A: top digit over the whole array (keys -> Y; argsort: indices generated)
     {
         const int rc = enqueue_lsd_pass<KD, PL>(ws, keys, vals, yk, yv, nullptr, n, 8 * (P - 1), flip, nan_last,
-                                                true, false, false, s);
+                                                true, false, false, s, nullptr, 0, false, nullptr, true);
         if (rc) return rc;
     }
     {
@@ -491,7 +510,7 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     // one segmented pass over a device-built table (src -> dst)
     auto seg_pass = [&](const char *const names[3], const HybTable &t, uint32_t *rsum, uint32_t *rpre,
                         uint32_t *stot, int shift, const void *sk, const void *sv, void *dk, void *dv,
-                        bool last) -> int {
+                        bool last, bool unstable) -> int {
         // the kernels walk the table's ranges with a bounded grid: a table
         // that turns out (nearly) empty costs few workgroups
         const uint32_t grid = std::min<uint32_t>(t.cap, MAX_RANGES);
@@ -515,7 +534,16 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
         a.canon_nan = last && canon_nan;
         SegRanges r{t.begin, t.end, t.sbase, t.bseg, rpre, stot, 0, t.cap, t.nr};
         KernelTimer kt(names[2], s);
-        if (nan_last)
+        // keys only: pass B's input order does not matter (it is an MSD
+        // pass inside each top bucket), so it ranks without keeping it
+        constexpr bool UN = PL == PL_NONE;
+        if (unstable && nan_last)
+            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, SS::BLOCK, SS::ITEMS, true, UN>), dim3(grid),
+                               dim3(SS::BLOCK), 0, s, a, r);
+        else if (unstable)
+            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, SS::BLOCK, SS::ITEMS, false, UN>), dim3(grid),
+                               dim3(SS::BLOCK), 0, s, a, r);
+        else if (nan_last)
             hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, SS::BLOCK, SS::ITEMS, true>), dim3(grid),
                                dim3(SS::BLOCK), 0, s, a, r);
         else
@@ -527,7 +555,7 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     // pass B: second digit inside every top bucket (Y -> X)
     static const char *const msd_names[3] = {"hyb_msd_upsweep", "hyb_msd_scan", "hyb_msd_downsweep"};
     static const char *const low_names[3] = {"hyb_large_upsweep", "hyb_large_scan", "hyb_large_downsweep"};
-    int rc = seg_pass(msd_names, h.tb, ws->rsum(), ws->rpre(), segtot, 8 * (P - 2), yk, yv, xk, xv, false);
+    int rc = seg_pass(msd_names, h.tb, ws->rsum(), ws->rpre(), segtot, 8 * (P - 2), yk, yv, xk, xv, false, true);
     if (rc) return rc;
     {
         KernelTimer t("hyb_tables", s);
@@ -564,7 +592,7 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     // last pass of argsort writes the indices)
     void *sk = xk, *sv = xv, *dk = yk, *dv = yv;
     for (int q = 0; q < P - 2; ++q) {
-        rc = seg_pass(low_names, h.td, rsum_d, rpre_d, segtot_d, 8 * q, sk, sv, dk, dv, q == P - 3);
+        rc = seg_pass(low_names, h.td, rsum_d, rpre_d, segtot_d, 8 * q, sk, sv, dk, dv, q == P - 3, q == 0);
         if (rc) return rc;
         std::swap(sk, dk);
         std::swap(sv, dv);

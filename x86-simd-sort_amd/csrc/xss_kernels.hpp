@@ -369,7 +369,7 @@ struct DownsweepSmem {
 // The tile holds keys [tbase, tbase + valid) (valid == TILE when FULL).
 // seg_base: start of the segment the tile belongs to (segment-local argsort
 // indices are position - seg_base).
-template <class KD, int PL, int BLOCK, int ITEMS, bool FULL, bool NL>
+template <class KD, int PL, int BLOCK, int ITEMS, bool FULL, bool NL, bool UNST = false>
 __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<KD, PL, BLOCK, ITEMS> &sm,
                                                uint64_t tbase, uint32_t valid_in, uint64_t seg_base, uint64_t &run)
 {
@@ -411,6 +411,20 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
     for (int i = 0; i < ITEMS; ++i) {
         const uint32_t lp = lbase + i * 64;
         const uint32_t d = (FULL || lp < valid) ? KD::digit(key[i], flip, NL, shift) : 255u;
+        if constexpr (UNST && FULL && PL == PL_NONE) {
+            // keys only, where any order inside a digit gives the same
+            // result: each lane reserves its own slot with one returning LDS
+            // add, no peer match.  A wave whose lanes share one digit (sorted
+            // runs) reserves its 64 slots with one add by lane 0, since
+            // same-address adds serialise; bit 24 marks it, and each lane's
+            // slot is then lane 0's base + lane (resolved after the barrier).
+            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+            const bool uni = __ballot(d != d0) == 0;
+            uint32_t old = 0;
+            if (!uni || lane == 0) old = atomicAdd(&wc[d], uni ? 64u : 1u);
+            rd[i] = old | (d << 16) | (uni ? (1u << 24) : 0u);
+            continue;
+        }
         const uint64_t peers = match_any8(d);
         const uint32_t below = lanes_below(peers);
 #if XSS_RANK_ATOMIC
@@ -463,8 +477,12 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
     // --- tile-local sort: scatter into LDS at slot base + rank
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-        const uint32_t d = rd[i] >> 16;
-        const uint32_t lpos = wc[d] + (rd[i] & 0xffffu);
+        uint32_t r = rd[i];
+        if constexpr (UNST && FULL && PL == PL_NONE) {
+            if (r >> 24) r = (__builtin_amdgcn_readfirstlane(r) + lane) & 0xffffffu;
+        }
+        const uint32_t d = r >> 16;
+        const uint32_t lpos = wc[d] + (r & 0xffffu);
         sm.keys[lpos] = key[i];
         sm.dig[lpos] = uint8_t(d);
         if constexpr (HAS_PL) sm.vals[lpos] = val[i];
@@ -559,7 +577,10 @@ struct ScatterWaves {
 #define XSS_SCATTER_OCCUPANCY(KD, PL, BLOCK) \
     __attribute__((amdgpu_waves_per_eu(ScatterWaves<KD, PL, BLOCK>::value)))
 
-template <class KD, int PL, int BLOCK, int ITEMS, bool NL>
+// UNST (keys only): full tiles rank without keeping the order inside a
+// digit; for a pass whose input order does not matter (the first LSD pass,
+// the MSD passes of the MSD-first sort).
+template <class KD, int PL, int BLOCK, int ITEMS, bool NL, bool UNST = false>
 __global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL, BLOCK) void downsweep_kernel(PassArgs a)
 {
     constexpr int TILE = BLOCK * ITEMS;
@@ -592,7 +613,7 @@ __global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL, BLOCK) void do
     const uint32_t nfull = min(tile1, b.n / TILE);
     uint32_t tile = tile0;
     for (; tile < nfull; ++tile)
-        downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(b, sm, uint64_t(tile) * TILE, TILE, 0, run);
+        downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL, UNST>(b, sm, uint64_t(tile) * TILE, TILE, 0, run);
     for (; tile < tile1; ++tile)
         downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(b, sm, uint64_t(tile) * TILE, b.n - tile * TILE, 0, run);
 }
@@ -726,7 +747,7 @@ struct SegShape {
     static constexpr int WAVES = BYTES == 4 ? 4 : 1;
 };
 
-template <class KD, int PL, int BLOCK, int ITEMS, bool NL>
+template <class KD, int PL, int BLOCK, int ITEMS, bool NL, bool UNST = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SegShape<KD, PL>::WAVES)))
 void seg_downsweep_kernel(PassArgs a, SegRanges r)
 {
@@ -743,7 +764,8 @@ void seg_downsweep_kernel(PassArgs a, SegRanges r)
         const uint32_t dstart = digit_starts(r.segtot + size_t(r.rbseg[b]) * 256, sm.wsum);
         uint64_t run = tid < 256 ? sbase + dstart + r.rpre[size_t(tid) * r.stride + b] : 0u;
         uint64_t t = rb;
-        for (; t + TILE <= re; t += TILE) downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(a, sm, t, TILE, sbase, run);
+        for (; t + TILE <= re; t += TILE)
+            downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL, UNST>(a, sm, t, TILE, sbase, run);
         if (t < re) downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, t, uint32_t(re - t), sbase, run);
     }
 }

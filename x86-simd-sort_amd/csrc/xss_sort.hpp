@@ -102,9 +102,21 @@ struct SortPlan {
 // for n.  PL_IDX: `first` generates the indices, `last` writes them to
 // idx_out.
 template <class KD, int PL, class PS>
-int launch_downsweep(const PassArgs &a, uint32_t ranges, bool nan_last, hipStream_t s)
+int launch_downsweep(const PassArgs &a, uint32_t ranges, bool nan_last, bool unstable, hipStream_t s)
 {
     KernelTimer t("downsweep", s);
+    if constexpr (PL == PL_NONE) {
+        if (unstable) {
+            if (nan_last)
+                hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, true, true>), dim3(ranges),
+                                   dim3(PS::BLOCK), 0, s, a);
+            else
+                hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, false, true>), dim3(ranges),
+                                   dim3(PS::BLOCK), 0, s, a);
+            XSS_HIP_CHECK(hipGetLastError());
+            return XSS_OK;
+        }
+    }
     if (nan_last)
         hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, true>), dim3(ranges), dim3(PS::BLOCK), 0,
                            s, a);
@@ -117,11 +129,13 @@ int launch_downsweep(const PassArgs &a, uint32_t ranges, bool nan_last, hipStrea
 
 // ran != null: a pass of a sort that skips constant digits (src/dst are
 // then the two buffers the keys ping-pong between, chosen on the device).
+// unstable: a keys-only pass whose input order does not matter may give up
+// the order inside a digit (cheaper ranking).
 template <class KD, int PL>
 int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *dst_k, void *dst_v,
                      uint64_t *idx_out, size_t n, int shift, uint64_t flip, bool nan_last, bool first, bool last,
                      bool canon_nan, hipStream_t s, uint32_t *ran = nullptr, uint32_t pass = 0, bool skip_ok = false,
-                     uint64_t *andor = nullptr)
+                     uint64_t *andor = nullptr, bool unstable = false)
 {
     using Plan = SortPlan<KD, PL>;
     const uint32_t nn = uint32_t(n);
@@ -148,8 +162,9 @@ int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *
     a.ranges = ranges; a.rpre = ws->rpre(); a.total = ws->total();
     a.first = first; a.last = last; a.canon_nan = canon_nan;
     a.ran = ran; a.pass = pass; a.skip_ok = skip_ok; a.andor = andor;
-    if (Plan::small_tiles(n)) return launch_downsweep<KD, PL, SmallPassShape<KD, PL>>(a, ranges, nan_last, s);
-    return launch_downsweep<KD, PL, PassShape<KD, PL>>(a, ranges, nan_last, s);
+    if (Plan::small_tiles(n))
+        return launch_downsweep<KD, PL, SmallPassShape<KD, PL>>(a, ranges, nan_last, unstable, s);
+    return launch_downsweep<KD, PL, PassShape<KD, PL>>(a, ranges, nan_last, unstable, s);
 }
 
 // Enqueue one sort of n keys (in place when keys_in == keys_out).
@@ -211,7 +226,8 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
             const bool last = p == P - 1;
             const bool canon_pass = last && canon && nan_last;
             const int rc = enqueue_lsd_pass<KD, PL>(ws, keys_out, vals_out, k0, v0, idx_out, n, 8 * p, flip, nan_last,
-                                                    p == 0, last, canon_pass, s, ran, uint32_t(p), !canon_pass, andor);
+                                                    p == 0, last, canon_pass, s, ran, uint32_t(p), !canon_pass, andor,
+                                                    p == 0);
             if (rc) return rc;
         }
         using V = typename PayloadT<PL>::type;
@@ -233,7 +249,8 @@ int enqueue_sort(Workspace *ws, char *scratch, const void *keys_in, void *keys_o
             dv = (p % 2 == 0) ? static_cast<void *>(v0) : vals_out;
         }
         const int rc = enqueue_lsd_pass<KD, PL>(ws, src_k, src_v, dk, dv, idx_out, n, 8 * p, flip, nan_last, p == 0,
-                                                p == P - 1, (p == P - 1) && canon && nan_last, s);
+                                                p == P - 1, (p == P - 1) && canon && nan_last, s, nullptr, 0, false,
+                                                nullptr, p == 0);
         if (rc) return rc;
         src_k = dk;
         src_v = dv;

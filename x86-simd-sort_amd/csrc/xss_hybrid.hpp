@@ -218,6 +218,19 @@ struct HybShape {
     static constexpr bool HAS_SMALL = BYTES > 4;
     static constexpr bool HAS_MID = BYTES <= 8;
 };
+// Tile shape of the segmented passes of this sort: the batch form's
+// (SegShape), except 4-byte keys, which take 1024 x 16 tiles as the
+// one-array pass does.  Pass B ranks without a peer match, and the longer
+// runs per digit then pay: 0.68 vs 0.75 ms per pass at 2^28 f32 (8K-key
+// tiles were the faster shape while every pass ranked stably).
+#ifndef XSS_HYB_SEG_BLOCK_C1
+#define XSS_HYB_SEG_BLOCK_C1 1024
+#endif
+template <class KD, int PL>
+struct HybSegShape {
+    static constexpr int BLOCK = SegShape<KD, PL>::BYTES == 4 ? XSS_HYB_SEG_BLOCK_C1 : SegShape<KD, PL>::BLOCK;
+    static constexpr int ITEMS = SegShape<KD, PL>::ITEMS;
+};
 // average bucket size below the SMALL capacity (about 8 standard deviations
 // of a uniform bucket's size at ~2000 keys)
 constexpr uint32_t HYB_SMALL_MARGIN = 384;
@@ -280,8 +293,8 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
         key[i] = ok ? keys[idx] : U(0);
         if constexpr (HAS_PL) val[i] = ok ? vals[idx] : V(0);
     }
-    // one LSD pass over the digit at `shift`; stable unless `unst` (keys
-    // only, first pass: its input order does not matter)
+    // one LSD pass over the digit at `shift`; stable unless `unst` (the
+    // first pass, whose input order does not matter; never for argsort)
     auto pass = [&](const int shift, const bool unst) {
         for (uint32_t i = tid; i < WAVES * 256; i += HYB_BLOCK) s_off[i] = 0;
         __syncthreads();  // counters cleared; the previous pass's reads of s_keys are done
@@ -297,7 +310,7 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
             if (uint32_t(i) < it) {
                 const uint32_t idx = lbase + i * 64;
                 const uint32_t d = idx < m ? KD::digit(key[i], flip, NL, shift) : 255u;
-                if (PL == PL_NONE && unst) {
+                if (PL != PL_IDX && unst) {
                     // each key reserves its own slot with one returning LDS
                     // add (a wave of one digit: 64 slots by lane 0, bit 24);
                     // padding takes no slot (bit 25), so the keys fill
@@ -348,7 +361,7 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
         for (int i = 0; i < ITEMS; ++i) {
             if (uint32_t(i) < it) {
                 uint32_t r = rd[i];
-                if (PL == PL_NONE && unst) {
+                if (PL != PL_IDX && unst) {
                     if (r >> 25) continue;
                     if (r >> 24) r = (__builtin_amdgcn_readfirstlane(r) + lane) & 0xffffffu;
                 }
@@ -402,9 +415,9 @@ struct HybPlan {
         const int sh = shape(n);
         return sh == 2 ? HybShape<KD, PL, 2>::CAP : sh == 1 ? HybShape<KD, PL, 1>::CAP : CAP;
     }
-    // tile shape of the segmented passes (8K-key tiles for 4-byte keys
-    // measured faster here than the one-array pass's 16K)
-    using SS = SegShape<KD, PL>;
+    // tile shape of the segmented passes (16K-key tiles for 4-byte keys,
+    // see HybSegShape)
+    using SS = HybSegShape<KD, PL>;
     static constexpr uint64_t SEG_TILE = uint64_t(SS::BLOCK) * SS::ITEMS;
     // pass B's table must fit the workspace's [256][MAX_RANGES] range tables
     static uint64_t rk_b(size_t n)
@@ -534,9 +547,11 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
         a.canon_nan = last && canon_nan;
         SegRanges r{t.begin, t.end, t.sbase, t.bseg, rpre, stot, 0, t.cap, t.nr};
         KernelTimer kt(names[2], s);
-        // keys only: pass B's input order does not matter (it is an MSD
-        // pass inside each top bucket), so it ranks without keeping it
-        constexpr bool UN = PL == PL_NONE;
+        // pass B's input order does not matter (it is an MSD pass inside
+        // each top bucket), so it ranks without keeping it; argsort keeps
+        // every pass stable (ties in input order, the identity for sorted
+        // input)
+        constexpr bool UN = PL != PL_IDX;
         if (unstable && nan_last)
             hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, SS::BLOCK, SS::ITEMS, true, UN>), dim3(grid),
                                dim3(SS::BLOCK), 0, s, a, r);

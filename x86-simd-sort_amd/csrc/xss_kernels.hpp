@@ -411,10 +411,10 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
     for (int i = 0; i < ITEMS; ++i) {
         const uint32_t lp = lbase + i * 64;
         const uint32_t d = (FULL || lp < valid) ? KD::digit(key[i], flip, NL, shift) : 255u;
-        if constexpr (UNST && FULL && PL == PL_NONE) {
-            // keys only, where any order inside a digit gives the same
-            // result: each lane reserves its own slot with one returning LDS
-            // add, no peer match.  A wave whose lanes share one digit (sorted
+        if constexpr (UNST && FULL && PL != PL_IDX) {
+            // keys only, or key-value (whose payloads inside a run of equal
+            // keys may come in any order): each lane reserves its own slot
+            // with one returning LDS add, no peer match.  A wave whose lanes share one digit (sorted
             // runs) reserves its 64 slots with one add by lane 0, since
             // same-address adds serialise; bit 24 marks it, and each lane's
             // slot is then lane 0's base + lane (resolved after the barrier).
@@ -478,7 +478,7 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         uint32_t r = rd[i];
-        if constexpr (UNST && FULL && PL == PL_NONE) {
+        if constexpr (UNST && FULL && PL != PL_IDX) {
             if (r >> 24) r = (__builtin_amdgcn_readfirstlane(r) + lane) & 0xffffffu;
         }
         const uint32_t d = r >> 16;
@@ -577,7 +577,7 @@ struct ScatterWaves {
 #define XSS_SCATTER_OCCUPANCY(KD, PL, BLOCK) \
     __attribute__((amdgpu_waves_per_eu(ScatterWaves<KD, PL, BLOCK>::value)))
 
-// UNST (keys only): full tiles rank without keeping the order inside a
+// UNST (not argsort): full tiles rank without keeping the order inside a
 // digit; for a pass whose input order does not matter (the first LSD pass,
 // the MSD passes of the MSD-first sort).
 template <class KD, int PL, int BLOCK, int ITEMS, bool NL, bool UNST = false>

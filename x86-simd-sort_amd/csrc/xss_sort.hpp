@@ -105,7 +105,7 @@ template <class KD, int PL, class PS>
 int launch_downsweep(const PassArgs &a, uint32_t ranges, bool nan_last, bool unstable, hipStream_t s)
 {
     KernelTimer t("downsweep", s);
-    if constexpr (PL == PL_NONE) {
+    if constexpr (PL != PL_IDX) {
         if (unstable) {
             if (nan_last)
                 hipLaunchKernelGGL((downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, true, true>), dim3(ranges),
@@ -129,8 +129,8 @@ int launch_downsweep(const PassArgs &a, uint32_t ranges, bool nan_last, bool uns
 
 // ran != null: a pass of a sort that skips constant digits (src/dst are
 // then the two buffers the keys ping-pong between, chosen on the device).
-// unstable: a keys-only pass whose input order does not matter may give up
-// the order inside a digit (cheaper ranking).
+// unstable: a pass whose input order does not matter may give up the order
+// inside a digit (cheaper ranking); argsort passes always keep it.
 template <class KD, int PL>
 int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *dst_k, void *dst_v,
                      uint64_t *idx_out, size_t n, int shift, uint64_t flip, bool nan_last, bool first, bool last,

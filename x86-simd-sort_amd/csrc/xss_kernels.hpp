@@ -139,11 +139,22 @@ __device__ __forceinline__ uint32_t moves_odd(const uint32_t *ran, uint32_t pass
     return __builtin_amdgcn_readfirstlane(moves_before(ran, pass) & 1u);
 }
 
+// Histogram copies of an upsweep workgroup (its waves share them round-robin;
+// more copies, fewer same-bank collisions of the LDS adds) and 16-byte loads
+// in flight per thread (build-time knobs).
+#ifndef XSS_UP_HISTS
+#define XSS_UP_HISTS 1
+#endif
+#ifndef XSS_UP_UNR
+#define XSS_UP_UNR 4
+#endif
 template <class KD>
 __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
 {
     using U = typename KD::U;
-    __shared__ uint32_t h[256];
+    constexpr int NH = XSS_UP_HISTS;
+    __shared__ uint32_t hh[NH * 256];
+    uint32_t *h = hh + ((threadIdx.x >> 6) % NH) * 256;
     // the buffer holding the keys: a.keys, or (a sort that skips constant
     // digits, after an odd number of moves) a.keys_alt.  Chosen by a load
     // from a 2-entry LDS table rather than a select on the pointers: the
@@ -169,7 +180,7 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
     const uint32_t nr = a.nr_dev ? *a.nr_dev : gridDim.x;
     for (uint32_t rg = blockIdx.x; rg < nr; rg += gridDim.x) {
         __syncthreads();  // the previous range's histogram was written out
-        for (int i = threadIdx.x; i < 256; i += UP_BLOCK) h[i] = 0;
+        for (int i = threadIdx.x; i < NH * 256; i += UP_BLOCK) hh[i] = 0;
         __syncthreads();
         const U flip = U(a.flip);
         const bool nl = a.nan_last != 0;
@@ -204,8 +215,8 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
                 }
             }
         };
-        // four loads in flight per thread before any is consumed
-        constexpr uint32_t UNR = 4;
+        // XSS_UP_UNR loads in flight per thread before any is consumed
+        constexpr uint32_t UNR = XSS_UP_UNR;
         uint32_t v = threadIdx.x;
         for (; v + (UNR - 1) * UP_BLOCK < nvec; v += UNR * UP_BLOCK) {
             uint4 q[UNR];
@@ -227,7 +238,12 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
             atomicAdd(&h[KD::digit(keys[i], flip, nl, shift)], 1u);
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < 256; i += UP_BLOCK) a.rsum[size_t(i) * a.ranges + rg] = h[i];
+        for (int i = threadIdx.x; i < 256; i += UP_BLOCK) {
+            uint32_t c = 0;
+#pragma unroll
+            for (int j = 0; j < NH; ++j) c += hh[j * 256 + i];
+            a.rsum[size_t(i) * a.ranges + rg] = c;
+        }
     }
     if (track) {
         atomicAnd(reinterpret_cast<unsigned long long *>(&s_andor[0]), (unsigned long long)t_and | ~(unsigned long long)U(~U(0)));

@@ -218,6 +218,8 @@ struct HybShape {
     static constexpr bool HAS_SMALL = BYTES > 4;
     static constexpr bool HAS_MID = BYTES <= 8;
 };
+constexpr uint32_t HYB_LARGE_GRID = 1024;  // workgroups of each large-bucket pass
+
 // Tile shape of the segmented passes of this sort: the batch form's
 // (SegShape), except 4-byte keys, which take 1024 x 16 tiles as the
 // one-array pass does.  Pass B ranks without a peer match, and the longer
@@ -523,10 +525,10 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     // one segmented pass over a device-built table (src -> dst)
     auto seg_pass = [&](const char *const names[3], const HybTable &t, uint32_t *rsum, uint32_t *rpre,
                         uint32_t *stot, int shift, const void *sk, const void *sv, void *dk, void *dv,
-                        bool last, bool unstable) -> int {
+                        bool last, bool unstable, uint32_t max_grid) -> int {
         // the kernels walk the table's ranges with a bounded grid: a table
         // that turns out (nearly) empty costs few workgroups
-        const uint32_t grid = std::min<uint32_t>(t.cap, MAX_RANGES);
+        const uint32_t grid = std::min<uint32_t>(t.cap, max_grid);
         UpArgs u{};
         u.keys = sk; u.n = nn; u.shift = uint32_t(shift); u.nan_last = nan_last; u.flip = flip;
         u.rsum = rsum; u.ranges = t.cap; u.range_begin = t.begin; u.range_end = t.end; u.nr_dev = t.nr;
@@ -570,7 +572,8 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     // pass B: second digit inside every top bucket (Y -> X)
     static const char *const msd_names[3] = {"hyb_msd_upsweep", "hyb_msd_scan", "hyb_msd_downsweep"};
     static const char *const low_names[3] = {"hyb_large_upsweep", "hyb_large_scan", "hyb_large_downsweep"};
-    int rc = seg_pass(msd_names, h.tb, ws->rsum(), ws->rpre(), segtot, 8 * (P - 2), yk, yv, xk, xv, false, true);
+    int rc = seg_pass(msd_names, h.tb, ws->rsum(), ws->rpre(), segtot, 8 * (P - 2), yk, yv, xk, xv, false, true,
+                      MAX_RANGES);
     if (rc) return rc;
     {
         KernelTimer t("hyb_tables", s);
@@ -604,10 +607,15 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     }
     XSS_HIP_CHECK(hipGetLastError());
     // low digits of the large buckets (X -> Y -> X ...; P - 2 is even; the
-    // last pass of argsort writes the indices)
+    // last pass of argsort writes the indices).  Their table is sized for
+    // the worst case (every key in a large bucket) but is usually empty
+    // (spread data), so the passes take HYB_LARGE_GRID persistent
+    // workgroups: 4 per CU still fill the GPU when the table is full, and
+    // an empty table costs a quarter of a 4096-workgroup launch.
     void *sk = xk, *sv = xv, *dk = yk, *dv = yv;
     for (int q = 0; q < P - 2; ++q) {
-        rc = seg_pass(low_names, h.td, rsum_d, rpre_d, segtot_d, 8 * q, sk, sv, dk, dv, q == P - 3, q == 0);
+        rc = seg_pass(low_names, h.td, rsum_d, rpre_d, segtot_d, 8 * q, sk, sv, dk, dv, q == P - 3, q == 0,
+                      HYB_LARGE_GRID);
         if (rc) return rc;
         std::swap(sk, dk);
         std::swap(sv, dv);

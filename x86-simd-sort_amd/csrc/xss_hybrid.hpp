@@ -225,13 +225,26 @@ constexpr uint32_t HYB_LARGE_GRID = 1024;  // workgroups of each large-bucket pa
 // one-array pass does.  Pass B ranks without a peer match, and the longer
 // runs per digit then pay: 0.68 vs 0.75 ms per pass at 2^28 f32 (8K-key
 // tiles were the faster shape while every pass ranked stably).
+// 5..8-byte items (64-bit keys, 32-bit argsort with its index) take
+// 1024 x 12 tiles, as the one-array pass does: argsort f32 2^28 4.60 ->
+// 4.46 ms, qsort u64 2^27 3.60 -> 3.57 ms (the batch form's 512 x 16
+// before).  Build-time knobs XSS_HYB_SEG_BLOCK_C2 / XSS_HYB_SEG_ITEMS_C2.
 #ifndef XSS_HYB_SEG_BLOCK_C1
 #define XSS_HYB_SEG_BLOCK_C1 1024
 #endif
+#ifndef XSS_HYB_SEG_BLOCK_C2
+#define XSS_HYB_SEG_BLOCK_C2 1024
+#endif
+#ifndef XSS_HYB_SEG_ITEMS_C2
+#define XSS_HYB_SEG_ITEMS_C2 12
+#endif
 template <class KD, int PL>
 struct HybSegShape {
-    static constexpr int BLOCK = SegShape<KD, PL>::BYTES == 4 ? XSS_HYB_SEG_BLOCK_C1 : SegShape<KD, PL>::BLOCK;
-    static constexpr int ITEMS = SegShape<KD, PL>::ITEMS;
+    static constexpr int BYTES = SegShape<KD, PL>::BYTES;
+    static constexpr int BLOCK = BYTES == 4   ? XSS_HYB_SEG_BLOCK_C1
+                               : BYTES <= 8 ? XSS_HYB_SEG_BLOCK_C2
+                                            : SegShape<KD, PL>::BLOCK;
+    static constexpr int ITEMS = (BYTES > 4 && BYTES <= 8) ? XSS_HYB_SEG_ITEMS_C2 : SegShape<KD, PL>::ITEMS;
 };
 // average bucket size below the SMALL capacity (about 8 standard deviations
 // of a uniform bucket's size at ~2000 keys)

@@ -103,6 +103,18 @@ XSS_DECLARE_T(i64, int64_t)
 XSS_DECLARE_T(f64, double)
 
 /*
+ * qsort<float> that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * semantics as xss_gpu_qsort_f32, but its scratch is a fixed device array
+ * inside the library (about 24 MiB per GPU, held until the process ends), so
+ * n is limited to 4,194,304 keys (larger n: XSS_ERR_INVALID).  Every call on
+ * one GPU uses that GPU's one array: calls that overlap on a GPU (several
+ * streams or threads at once) race, and the caller must serialise them.
+ */
+XSS_API int xss_gpu_qsort_f32_noalloc(float *arr, size_t n, int hasnan, int descending, xss_stream_t stream);
+
+/*
  * NumPy-style arg forms (x86simdsortStatic::argsort(T *, size_t *arg, size,
  * hasnan, descending) and argselect(T *, size_t *arg, k, size, hasnan),
  * src/x86simdsort-static-incl.h:31-36,43-46,92-97,108-113).  `arg` holds n

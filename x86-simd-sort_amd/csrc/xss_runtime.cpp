@@ -142,7 +142,10 @@ hipEvent_t pooled_event()
 }
 }  // namespace
 
-bool timing_on() { return g_timing; }
+thread_local int g_timing_quiet = 0;
+bool timing_on() { return g_timing && g_timing_quiet == 0; }
+TimingSuppressed::TimingSuppressed() { ++g_timing_quiet; }
+TimingSuppressed::~TimingSuppressed() { --g_timing_quiet; }
 
 void timing_begin(const char *name, hipStream_t s)
 {

@@ -55,6 +55,12 @@ int set_error(int code, const char *fmt, ...);
 // Optional per-kernel timing (xss_gpu_timing_enable): HIP events recorded
 // on the launch stream around each kernel, summed per kernel name.
 bool timing_on();
+// While an instance lives, the calling thread records no timing events
+// (entry points that must not create runtime objects).
+struct TimingSuppressed {
+    TimingSuppressed();
+    ~TimingSuppressed();
+};
 void timing_begin(const char *name, hipStream_t s);
 void timing_end(hipStream_t s);
 struct KernelTimer {

@@ -1,133 +1,107 @@
 """Parity at the full workload sizes: 2^28 32-bit keys, 2^27 64-bit keys,
-2^29 16-bit keys (1 GiB of keys each), the five input recipes (uniform over
-the full bit range, 1024 distinct values, sorted, reverse sorted, 1% NaN with
-hasnan), ascending and descending.
+2^29 16-bit keys (1 GiB of keys each).
 
-At these sizes the oracle's single-threaded restatement takes minutes, so
-results are checked through properties that fix a sort's output:
-  * sortedness under the reference's order (tests/test-qsort-common.h:29-36),
-    every NaN one class placed last (first when descending), -0.0 == +0.0;
-  * equality of the output's value classes with the input's, sorted (a
-    permutation), and the same count of each zero's bit pattern;
-  * qsort(hasnan) writes only the canonical NaN (src/xss-custom-float.h:34-38,81-85);
-  * argsort / keyvalue_qsort with values = original index: the indices are a
-    permutation and gather the output keys from the input;
-  * select / partial sort: the k-th value equals numpy's partition of the
-    value classes, with the partition property around it.
-Where the reference library was built (oracle/_ref), the 32-bit sorts are
-also compared with the reference's own output on the same input.
+  * qsort and argsort of every key type on every recipe of the workload
+    (uniform over the full bit range, 1024 distinct values, sorted, reverse
+    sorted, and for floats 1% NaN with hasnan), ascending and descending;
+  * keyvalue_qsort of u64 Zipf(1.1) keys with their index as payload (2^27)
+    and of normal float keys with u32 indices (2^28), plus 1024-distinct and
+    sorted keys that take the MSD-first route; keyvalue_select at k = N/2,
+    N-1 and keyvalue_partial_sort at k = 1000, N/100 on the same arrays;
+  * qselect at k = N/2, N-1 and partial_qsort at k = 1000, N/100;
+  * qsort of 1,000,000 floats uniform in [-1e6, 1e6], the plumbing
+    workload, bit for bit against the oracle (and the reference where built).
+
+Inputs are drawn and results checked on the GPU (xss_devcheck): an output
+is correct when its order keys equal the input's order keys sorted by
+torch.sort (ordered, and a permutation), with the -0.0 count and the
+canonical NaN checked bit-wise; argsort and key-value results must gather
+their keys from the input through a permutation of indices.  The uniform
+recipe of every qsort is also compared with the reference's own output where
+the reference library was built (oracle/_ref).
 """
+import functools
+
 import numpy as np
 import pytest
+import torch
 
+import xss_devcheck as D
 import xss_testlib as T
 
 pytestmark = pytest.mark.gpu
 
-N_OF = {2: 1 << 29, 4: 1 << 28, 8: 1 << 27}
+N_OF = {16: 1 << 29, 32: 1 << 28, 64: 1 << 27}
 RECIPES = ["random", "dups", "sorted", "reverse", "nan"]
+CELLS = [(t, r) for t in T.TAGS for r in RECIPES if r != "nan" or t in T.FLOAT_TAGS]
+NP_OF = {tag: np.dtype(dt) for tag, dt in T.TAGS.items()}
 
 
-def n_for(tag):
-    return N_OF[np.dtype(T.TAGS[tag]).itemsize]
+def darray(xss, x, tag):
+    return xss.DArray(tag, x.numel(), x.view(torch.uint8))
 
 
-def ordv(a, tag, desc=False):
-    """Order key of the reference's comparator as unsigned integers: value
-    classes (every NaN one class above +inf, -0 == +0) mapped monotonically."""
-    dt = np.dtype(T.TAGS[tag])
-    u = T.UINT[dt.itemsize]
-    nb = dt.itemsize * 8
-    sign = u(1) << u(nb - 1)
-    b = T.bits(np.ascontiguousarray(a)).copy()
-    if tag in T.FLOAT_TAGS:
-        b[b == sign] = 0                          # -0 -> +0
-        nan = T.is_nan(a, tag)
-        neg = (b & sign) != 0
-        b = np.where(neg, ~b, b | sign)
-        b[nan] = u(~u(0))                         # all NaNs above +inf
-    elif tag.startswith("i"):
-        b ^= sign
-    if desc:
-        b = ~b
-    return b
+def host(x, tag):
+    return x.cpu().numpy().view(NP_OF[tag])
 
 
-def check_sort(got, inp, tag, hasnan, desc, canon_nan, what):
-    og = ordv(got, tag, desc)
-    assert (og[1:] >= og[:-1]).all(), f"{what}: not sorted"
-    assert np.array_equal(og, np.sort(ordv(inp, tag, desc))), f"{what}: not a permutation of the input"
-    if tag in T.FLOAT_TAGS:
-        nb = got.dtype.itemsize * 8
-        negz = T.UINT[got.dtype.itemsize](1) << T.UINT[got.dtype.itemsize](nb - 1)
-        assert (T.bits(got) == negz).sum() == (T.bits(inp) == negz).sum(), f"{what}: -0.0 count differs"
-        if hasnan and canon_nan:
-            nanbits = T.bits(got)[T.is_nan(got, tag)]
-            assert (nanbits == T.QNAN[tag]).all(), f"{what}: NaN not canonical"
-
-
-def gather_ok(keys_out, inp, idx, what):
-    n = inp.size
-    idx = idx.astype(np.int64)
-    assert idx.min() >= 0 and idx.max() < n, f"{what}: index out of range"
-    seen = np.zeros(n, bool)
-    seen[idx] = True
-    assert seen.all(), f"{what}: indices are not a permutation"
-    assert np.array_equal(T.bits(inp[idx]), T.bits(keys_out)), f"{what}: indices do not gather the keys"
-
-
-CASES_32 = [("f32", r) for r in RECIPES] + [("u32", r) for r in RECIPES[:4]] + [("i32", "random")]
-CASES_64_16 = [(t, "random") for t in ("f64", "u64", "i64", "f16", "u16", "i16")] + [("f64", "nan"), ("f16", "nan")]
+@pytest.fixture(autouse=True)
+def _free():
+    yield
+    torch.cuda.empty_cache()
 
 
 @pytest.mark.parametrize("desc", [False, True])
-@pytest.mark.parametrize("tag,recipe", CASES_32)
-def test_qsort_32bit_full(xss, tag, recipe, desc):
-    n = n_for(tag)
-    inp = T.gen(tag, recipe, n, 4242)
+@pytest.mark.parametrize("tag,recipe", CELLS)
+def test_qsort_full(xss, tag, recipe, desc):
+    n = N_OF[D.WIDTH[tag]]
+    inp = D.gen(tag, recipe, n, 4242 + len(recipe))
     hasnan = recipe == "nan"
-    d = xss.DArray.from_numpy(inp)
-    xss.qsort(d, hasnan, desc)
-    got = d.numpy()
-    del d
-    what = f"qsort<{tag}> {recipe} n=2^28 desc={desc}"
-    check_sort(got, inp, tag, hasnan, desc, True, what)
+    got = inp.clone()
+    what = f"qsort<{tag}> {recipe} n={n} desc={desc}"
+    print(what, flush=True)
+    xss.qsort(darray(xss, got, tag), hasnan, desc)
+    D.check_sort(got, inp, tag, hasnan, desc, True, what)
     R = T.REF()
-    if R is not None:
-        exp = R.qsort(inp, tag, hasnan, desc)
-        if not np.array_equal(T.bits(got), T.bits(exp)):
-            T.assert_sorted_equal(got, exp, tag, what + " vs reference", zeros_as_values=True)
+    if R is not None and recipe in ("random", "nan"):
+        hi, hg = host(inp, tag), host(got, tag)
+        exp = R.qsort(hi, tag, hasnan, desc)
+        if not np.array_equal(T.bits(hg), T.bits(exp)):
+            # The reference is compared only where its own output is ordered:
+            # its _Float16 qsort on CPUs without AVX512-FP16 (the emulated
+            # avx512_icl path) returns an unordered permutation for random
+            # arrays of 2^25 keys and more.  The result above is already
+            # checked in full against the input.
+            ref_dev = torch.from_numpy(exp.view(T.UINT[exp.dtype.itemsize]).view(
+                {2: np.int16, 4: np.int32, 8: np.int64}[exp.dtype.itemsize])).cuda()
+            try:
+                D.check_sorted_perm(ref_dev, inp, tag, desc, "reference output")
+            except AssertionError as e:
+                print(f"{what}: reference output is not ordered, not compared ({e})", flush=True)
+                return
+            T.assert_sorted_equal(hg, exp, tag, what + " vs reference", zeros_as_values=True)
 
 
 @pytest.mark.parametrize("desc", [False, True])
-@pytest.mark.parametrize("tag,recipe", CASES_64_16)
-def test_qsort_64_16bit_full(xss, tag, recipe, desc):
-    n = n_for(tag)
-    inp = T.gen(tag, recipe, n, 4343)
+@pytest.mark.parametrize("tag,recipe", CELLS)
+def test_argsort_full(xss, tag, recipe, desc):
+    n = N_OF[D.WIDTH[tag]]
+    inp = D.gen(tag, recipe, n, 4444 + len(recipe))
     hasnan = recipe == "nan"
-    d = xss.DArray.from_numpy(inp)
-    xss.qsort(d, hasnan, desc)
-    got = d.numpy()
-    del d
-    check_sort(got, inp, tag, hasnan, desc, True, f"qsort<{tag}> {recipe} n={n} desc={desc}")
+    keep = inp.clone()
+    what = f"argsort<{tag}> {recipe} n={n} desc={desc}"
+    print(what, flush=True)
+    idx = xss.argsort(darray(xss, inp, tag), hasnan, desc)
+    assert torch.equal(inp, keep), f"{what}: argsort modified its input"
+    i64 = idx.t[: 8 * n].view(torch.int64)
+    D.check_permutation(i64, n, what)
+    D.check_sort(inp[i64], inp, tag, hasnan, desc, False, what)
+    if recipe == "sorted" and not desc:  # already in order: the identity (src/xss-common-argsort.h:622-625)
+        assert torch.equal(i64, torch.arange(n, device="cuda")), f"{what}: not the identity"
 
 
-@pytest.mark.parametrize("desc", [False, True])
-@pytest.mark.parametrize("tag", ["f32", "u64", "f16"])
-def test_argsort_full(xss, tag, desc):
-    n = n_for(tag)
-    inp = T.gen(tag, "nan" if tag in T.FLOAT_TAGS else "dups", n, 4444)
-    hasnan = tag in T.FLOAT_TAGS
-    d = xss.DArray.from_numpy(inp)
-    idx = xss.argsort(d, hasnan, desc).numpy()
-    del d
-    what = f"argsort<{tag}> n={n} desc={desc}"
-    keys = inp[idx.astype(np.int64)]
-    gather_ok(keys, inp, idx, what)
-    check_sort(keys, inp, tag, hasnan, desc, False, what)
-
-
-def zipf_keys(n, seed):
+@functools.lru_cache(maxsize=1)
+def _zipf_host(n, seed):
     rng = np.random.default_rng(seed)
     z = rng.zipf(1.1, n).astype(np.uint64)
     bad = z >= (1 << 32)
@@ -137,60 +111,109 @@ def zipf_keys(n, seed):
     return z
 
 
-@pytest.mark.parametrize("desc", [False, True])
-def test_keyvalue_u64_zipf_full(xss, desc):
-    n = 1 << 27
-    k = zipf_keys(n, 11)
-    v = np.arange(n, dtype=np.uint64)
-    dk, dv = xss.DArray.from_numpy(k), xss.DArray.from_numpy(v)
-    xss.keyvalue_qsort(dk, dv, False, desc)
-    gk, gv = dk.numpy(), dv.numpy()
-    del dk, dv
-    what = f"keyvalue_qsort<u64,u64> zipf n=2^27 desc={desc}"
-    gather_ok(gk, k, gv, what)
-    check_sort(gk, k, "u64", False, desc, False, what)
+def zipf_keys(n, seed):
+    """Zipf(s = 1.1) ranks below 2^32 (seeded, drawn once per session)."""
+    return torch.from_numpy(_zipf_host(n, seed).view(np.int64)).cuda()
+
+
+def kv_input(kind, n):
+    """(key tag, value tag, keys) of the key-value workloads."""
+    if kind == "zipf":
+        return "u64", "u64", zipf_keys(n, 11)
+    if kind == "normal":
+        g = torch.Generator(device="cuda")
+        g.manual_seed(12)
+        return "f32", "u32", torch.randn(n, device="cuda", generator=g).view(torch.int32)
+    tag = "u64" if kind.endswith("64") else "f32"
+    return tag, "u64" if tag == "u64" else "u32", D.gen(tag, kind.split("_")[0], n, 13)
+
+
+KV_KINDS = ["zipf", "normal", "dups_64", "sorted_64", "dups_32", "sorted_32"]
+
+
+def kv_n(kind):
+    return 1 << 27 if kind in ("zipf", "dups_64", "sorted_64") else 1 << 28
 
 
 @pytest.mark.parametrize("desc", [False, True])
-def test_keyvalue_f32_normal_full(xss, desc):
-    n = 1 << 28
-    rng = np.random.default_rng(12)
-    k = rng.standard_normal(n, dtype=np.float32)
-    v = np.arange(n, dtype=np.uint32)
-    dk, dv = xss.DArray.from_numpy(k), xss.DArray.from_numpy(v)
-    xss.keyvalue_qsort(dk, dv, False, desc)
-    gk, gv = dk.numpy(), dv.numpy()
-    del dk, dv
-    what = f"keyvalue_qsort<f32,u32> normal n=2^28 desc={desc}"
-    gather_ok(gk, k, gv, what)
-    check_sort(gk, k, "f32", False, desc, False, what)
+@pytest.mark.parametrize("kind", KV_KINDS)
+def test_keyvalue_qsort_full(xss, kind, desc):
+    n = kv_n(kind)
+    ktag, vtag, k = kv_input(kind, n)
+    v = torch.arange(n, device="cuda", dtype=torch.int64)
+    if vtag == "u32":
+        v = v.to(torch.int32)
+    gk, gv = k.clone(), v.clone()
+    what = f"keyvalue_qsort<{ktag},{vtag}> {kind} n={n} desc={desc}"
+    print(what, flush=True)
+    xss.keyvalue_qsort(darray(xss, gk, ktag), darray(xss, gv, vtag), False, desc)
+    D.check_gather(gk, k, gv.to(torch.int64), what)
+    D.check_sort(gk, k, ktag, False, desc, False, what)
 
 
-@pytest.mark.parametrize("tag", ["f32", "u64"])
-def test_select_partial_full(xss, tag):
-    n = n_for(tag)
-    inp = T.gen(tag, "nan" if tag == "f32" else "random", n, 4545)
-    hasnan = tag == "f32"
-    o_in = ordv(inp, tag)
+@pytest.mark.parametrize("kind", ["zipf", "normal"])
+def test_keyvalue_select_partial_full(xss, kind):
+    n = kv_n(kind)
+    ktag, vtag, k = kv_input(kind, n)
+    v = torch.arange(n, device="cuda", dtype=torch.int64 if vtag == "u64" else torch.int32)
+    srt = torch.sort(D.order_key(k, ktag)).values
+    for kk in (n // 2, n - 1):
+        gk, gv = k.clone(), v.clone()
+        what = f"keyvalue_select<{ktag},{vtag}> {kind} n={n} k={kk}"
+        print(what, flush=True)
+        xss.keyvalue_select(darray(xss, gk, ktag), darray(xss, gv, vtag), kk, False, False)
+        D.check_gather(gk, k, gv.to(torch.int64), what)
+        og = D.order_key(gk, ktag)
+        assert og[kk] == srt[kk], f"{what}: k-th key differs"
+        assert bool((og[:kk] <= srt[kk]).all()) and bool((og[kk + 1:] >= srt[kk]).all()), f"{what}: partition"
+    for kk in (1000, n // 100):
+        gk, gv = k.clone(), v.clone()
+        what = f"keyvalue_partial_sort<{ktag},{vtag}> {kind} n={n} k={kk}"
+        print(what, flush=True)
+        xss.keyvalue_partial_sort(darray(xss, gk, ktag), darray(xss, gv, vtag), kk, False, False)
+        D.check_gather(gk, k, gv.to(torch.int64), what)
+        og = D.order_key(gk, ktag)
+        assert torch.equal(og[:kk], srt[:kk]), f"{what}: sorted prefix differs"
+        assert bool((og[kk:] >= srt[kk - 1]).all()), f"{what}: partition"
+
+
+@pytest.mark.parametrize("tag,recipe", [("f32", "nan"), ("u32", "random"), ("u64", "random"), ("f64", "nan"),
+                                        ("i16", "dups"), ("f16", "nan")])
+def test_select_partial_full(xss, tag, recipe):
+    n = N_OF[D.WIDTH[tag]]
+    inp = D.gen(tag, recipe, n, 4545)
+    hasnan = recipe == "nan"
+    srt = torch.sort(D.order_key(inp, tag)).values
     for k in (n // 2, n - 1):
-        d = xss.DArray.from_numpy(inp)
-        xss.qselect(d, k, hasnan)
-        got = d.numpy()
-        del d
-        what = f"qselect<{tag}> n={n} k={k}"
-        og = ordv(got, tag)
-        kth = np.partition(o_in, k)[k]
-        assert og[k] == kth, f"{what}: k-th value differs"
-        assert (og[:k] <= kth).all() and (og[k + 1:] >= kth).all(), f"{what}: partition property"
-        assert np.array_equal(np.sort(T.bits(got)), np.sort(T.bits(inp))), f"{what}: not a permutation"
+        got = inp.clone()
+        what = f"qselect<{tag}> {recipe} n={n} k={k}"
+        print(what, flush=True)
+        xss.qselect(darray(xss, got, tag), k, hasnan)
+        og = D.order_key(got, tag)
+        assert og[k] == srt[k], f"{what}: k-th value differs"
+        assert bool((og[:k] <= srt[k]).all()) and bool((og[k + 1:] >= srt[k]).all()), f"{what}: partition"
+        assert torch.equal(torch.sort(got).values, torch.sort(inp).values), f"{what}: not a permutation"
     for k in (1000, n // 100):
-        d = xss.DArray.from_numpy(inp)
-        xss.partial_qsort(d, k, hasnan)
-        got = d.numpy()
-        del d
-        what = f"partial_qsort<{tag}> n={n} k={k}"
-        og = ordv(got, tag)
-        exp = np.sort(np.partition(o_in, k - 1)[:k])
-        assert np.array_equal(og[:k], exp), f"{what}: sorted prefix differs"
-        assert (og[k:] >= og[k - 1]).all(), f"{what}: partition property"
-        assert np.array_equal(np.sort(T.bits(got)), np.sort(T.bits(inp))), f"{what}: not a permutation"
+        got = inp.clone()
+        what = f"partial_qsort<{tag}> {recipe} n={n} k={k}"
+        print(what, flush=True)
+        xss.partial_qsort(darray(xss, got, tag), k, hasnan)
+        og = D.order_key(got, tag)
+        assert torch.equal(og[:k], srt[:k]), f"{what}: sorted prefix differs"
+        assert bool((og[k:] >= srt[k - 1]).all()), f"{what}: partition"
+        assert torch.equal(torch.sort(got).values, torch.sort(inp).values), f"{what}: not a permutation"
+
+
+def test_plumbing_config_1m(xss):
+    """qsort<float> of N = 1,000,000 uniform in [-1e6, 1e6], no NaN,
+    ascending: bit for bit against the oracle and the reference."""
+    rng = np.random.default_rng(20240601)
+    a = rng.uniform(-1e6, 1e6, 1_000_000).astype(np.float32)
+    d = xss.DArray.from_numpy(a)
+    xss.qsort(d)
+    got = d.numpy()
+    exp = T.ORACLE().qsort(a, "f32")
+    assert np.array_equal(T.bits(got), T.bits(exp)), "qsort<float> 1M differs from the oracle"
+    R = T.REF()
+    if R is not None:
+        T.assert_sorted_equal(got, R.qsort(a, "f32"), "f32", "1M vs reference", zeros_as_values=True)

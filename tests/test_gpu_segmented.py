@@ -124,3 +124,98 @@ def test_segmented_graph_replay(xss):
         other = xss.DArray.from_numpy(T.gen("u32", "random", 5000, rep))
         xss.qsort(other)
         torch.cuda.synchronize()
+
+
+# ---------------------------------------------------------------- the batch workload at full size
+
+def c4_offsets():
+    """8192 segments, lengths log-uniform in [16, 2^20] (~760M keys), the
+    batch workload's seeded lengths."""
+    rng = np.random.default_rng(2024)
+    sizes = np.exp(rng.uniform(np.log(16), np.log(1 << 20), 8192)).astype(np.int64)
+    return offsets_of(sizes)
+
+
+@pytest.fixture(scope="module")
+def c4(xss):
+    import torch
+    import xss_devcheck as D
+    off = c4_offsets()
+    n = int(off[-1])
+    seg = D.seg_ids(off, "cuda")
+    yield off, n, seg
+    del seg
+    torch.cuda.empty_cache()
+
+
+def _c4_floats(n):
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(77)
+    v = torch.rand(n, device="cuda", generator=g) * 2 - 1
+    v[torch.rand(n, device="cuda", generator=g) < 0.001] = float("nan")
+    return v.view(torch.int32)
+
+
+def test_segmented_c4_qsort_all_segments(xss, c4):
+    """qsort of all 8192 segments (U(-1,1) floats, 0.1% NaN, hasnan): every
+    segment ordered and a permutation of its input, NaNs canonical."""
+    import torch
+    import xss_devcheck as D
+    off, n, seg = c4
+    inp = _c4_floats(n)
+    for desc in (False, True):
+        got = inp.clone()
+        Plan(xss, "qsort", "f32", off).run(xss.DArray("f32", n, got.view(torch.uint8)), hasnan=True, desc=desc)
+        D.check_segments_sorted_perm(got, inp, seg, "f32", desc, f"c4 qsort desc={desc}")
+        nan = D.is_nan(got, "f32")
+        assert int(nan.sum()) == int(D.is_nan(inp, "f32").sum()) and bool((got[nan] == 0x7FC00000).all())
+
+
+def test_segmented_c4_argsort_all_segments(xss, c4):
+    """argsort of int32 keys (full range) in all 8192 segments: segment-local
+    uint32 indices that gather each segment's keys in order."""
+    import torch
+    import xss_devcheck as D
+    off, n, seg = c4
+    g = torch.Generator(device="cuda")
+    g.manual_seed(78)
+    inp = torch.randint(-2 ** 31, 2 ** 31, (n,), dtype=torch.int32, device="cuda", generator=g)
+    keep = inp.clone()
+    idx = torch.empty(n, dtype=torch.int32, device="cuda")
+    Plan(xss, "argsort", "i32", off).run(xss.DArray("i32", n, inp.view(torch.uint8)),
+                                         xss.DArray("u32", n, idx.view(torch.uint8)))
+    assert torch.equal(inp, keep), "argsort modified the keys"
+    start = torch.as_tensor(off[:-1].astype(np.int64), device="cuda")
+    lens = torch.as_tensor((off[1:] - off[:-1]).astype(np.int64), device="cuda")
+    local = idx.to(torch.int64)
+    assert bool((local >= 0).all()) and bool((local < lens[seg]).all()), "index outside its segment"
+    glob = local + start[seg]
+    D.check_permutation(glob, n, "c4 argsort")
+    D.check_segments_sorted_perm(inp[glob], inp, seg, "i32", False, "c4 argsort gathered keys")
+
+
+def test_segmented_c4_qselect_all_segments(xss, c4):
+    """qselect at k = len/2 in all 8192 segments: the k-th value class at k,
+    nothing larger before it and nothing smaller after, a permutation."""
+    import torch
+    import xss_devcheck as D
+    off, n, seg = c4
+    inp = _c4_floats(n)
+    got = inp.clone()
+    Plan(xss, "qselect", "f32", off).run(xss.DArray("f32", n, got.view(torch.uint8)), hasnan=True)
+    start = torch.as_tensor(off[:-1].astype(np.int64), device="cuda")
+    lens = torch.as_tensor((off[1:] - off[:-1]).astype(np.int64), device="cuda")
+    seg64 = seg.to(torch.int64)
+    comp = lambda x: (seg64 << 32) | (D.order_key(x, "f32").to(torch.int64) + (1 << 31))  # noqa: E731
+    srt = torch.sort(comp(inp)).values
+    cg = comp(got)
+    assert torch.equal(torch.sort(cg).values, srt), "segments are not permutations of their inputs"
+    kpos = start + lens // 2
+    kth = srt[kpos]
+    assert torch.equal(cg[kpos], kth), "k-th value differs"
+    pos = torch.arange(n, device="cuda")
+    left = pos < kpos[seg64]
+    assert bool((cg[left] <= kth[seg64[left]]).all()), "a larger value left of k"
+    assert bool((cg[~left] >= kth[seg64[~left]]).all()), "a smaller value right of k"
+    assert torch.equal(torch.sort(got).values, torch.sort(inp).values), "NaN bits or keys changed"

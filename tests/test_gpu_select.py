@@ -64,16 +64,20 @@ def test_argselect(xss, tag):
         assert np.array_equal(T.bits(d.numpy()), T.bits(a)), "argselect modified its input"
 
 
-KV = [("u32", "u32"), ("f32", "u64"), ("u64", "u64"), ("f64", "i32"), ("i64", "f32"), ("i32", "f64")]
+KV = [(k, v) for k in T.KV_TAGS for v in T.KV_TAGS]  # the 36 pairs (lib/x86simdsort.cpp:280-293)
+KV_DISTS = ["random", "dups", "constant", "signed_zeros", "nan", "rand_with_max_and_nan", "sorted"]
 
 
 @pytest.mark.parametrize("desc", [False, True])
 @pytest.mark.parametrize("ktag,vtag", KV)
 def test_keyvalue_select_partial(xss, ktag, vtag, desc):
+    """keyvalue_select: the k-th key and the partition property, pairs kept;
+    keyvalue_partial_sort: the sorted prefix with its payloads checked as
+    multisets per run of equal keys (tests/test-keyvalue.cpp:102-171)."""
     O = T.ORACLE()
-    for seed, (dist, n) in enumerate((d, n) for d in DISTS for n in [3, 4097, 100003]):
+    for seed, (dist, n) in enumerate((d, n) for d in KV_DISTS for n in [3, 4097, 30011]):
         k = T.gen(ktag, dist, n, 8000 + seed)
-        v = np.arange(n).astype(T.TAGS[vtag])
+        v = T.gen(vtag, "random", n, 9000 + seed) if seed % 2 else np.arange(n).astype(T.TAGS[vtag])
         hasnan = dist in T.NAN_DISTS
         ek, ev = O.keyvalue_qsort(k, v, ktag, vtag, hasnan, desc)
         for kk in ks(n):
@@ -82,13 +86,10 @@ def test_keyvalue_select_partial(xss, ktag, vtag, desc):
             gk, gv = dk.numpy(), dv.numpy()
             what = f"kvselect<{ktag},{vtag}> {dist} n={n} k={kk} desc={desc}"
             T.assert_selected(gk, k, ek, kk, ktag, what)
-            # pairs stay together: each (key, value) of the output is an input pair
-            pin = np.lexsort((T.bits(v), T.bits(k)))
-            pout = np.lexsort((T.bits(gv), T.bits(gk)))
-            assert np.array_equal(T.bits(k)[pin], T.bits(gk)[pout]) and np.array_equal(T.bits(v)[pin], T.bits(gv)[pout]), what
-        for kk in sorted({1, min(n, 1000), n}):
+            T.assert_pairs_preserved(gk, gv, k, v, what)
+        for kk in sorted({1, 2, min(n, 1000), n // 2, n}):
             dk, dv = xss.DArray.from_numpy(k), xss.DArray.from_numpy(v)
             xss.keyvalue_partial_sort(dk, dv, kk, hasnan, desc)
             gk, gv = dk.numpy(), dv.numpy()
             what = f"kvpartial<{ktag},{vtag}> {dist} n={n} k={kk} desc={desc}"
-            T.assert_prefix_equal(gk, ek, kk, ktag, what)
+            T.assert_kv_partial(gk, gv, kk, k, v, ek, ev, ktag, what)

@@ -6,15 +6,21 @@ flags), keyvalue_qsort, qselect and partial_qsort requests is served three
 ways -- a burst into 4 streams with coalescing, a 2-stream server whose
 admission budget holds only a few requests at a time, and one request at a
 time -- and compared with the same requests run alone:
-  * qsort / keyvalue_qsort: bit-exact (both paths are stable, and qsort(hasnan)
-    writes the canonical NaN);
+  * qsort: bit-exact (a keys-only result is fixed by its keys, and
+    qsort(hasnan) writes the canonical NaN);
+  * keyvalue_qsort: keys bit-exact, payloads as a multiset within each run
+    of equal keys (the reference's kv sort is not stable either);
   * qselect: the same value class at k, the same multisets left and right of
     k, and a permutation of the input (the order inside a partition is not
     fixed);
   * partial_qsort: the sorted prefix as the partial-sort checker compares it,
     and the same multiset overall.
-The alone results themselves are checked against the oracle by the sort and
-select test files.
+Every served result is also checked against the oracle with the reference's
+checker rules.  A trace shaped like the served workload (70% qsort<float>
+of 1K-16M keys, 20% keyvalue_qsort<u64,u64> of Zipf keys of 1K-4M, 10%
+qselect / partial_qsort<u32> of 1K-16M, Poisson arrivals at 10k req/s) is
+checked through the full-size properties on the GPU, with the oracle for the
+requests of at most 2^20 keys.
 """
 import numpy as np
 import pytest
@@ -98,15 +104,38 @@ def _serve(xss, mix, **server_args):
     return results, stats
 
 
+def _oracle_check(item, sk, sv, w):
+    """The served result against the oracle, with the reference's checker
+    rules (tests/test-qsort-common.h:29-104, tests/test-keyvalue.cpp:44-100)."""
+    op, tag, vtag, keys, vals, k, hasnan, desc = item
+    O = T.ORACLE()
+    if op == "qsort":
+        T.assert_sorted_equal(sk, O.qsort(keys, tag, hasnan, desc), tag, w + " vs oracle")
+    elif op == "keyvalue_qsort":
+        ek, ev = O.keyvalue_qsort(keys, vals, tag, vtag, hasnan, desc)
+        T.assert_kv_equal(sk, sv, ek, ev, tag, w + " vs oracle")
+    elif op == "qselect":
+        if keys.size:
+            T.assert_selected(sk, keys, O.qselect(keys, tag, 0, hasnan, desc), k, tag, w + " vs oracle")
+    else:
+        exp = O.partial_qsort(keys, tag, k, hasnan, desc)
+        T.assert_prefix_equal(sk, exp, k, tag, w + " vs oracle")
+        assert np.array_equal(np.sort(T.vclass(sk, tag)), np.sort(T.vclass(exp, tag))), w + " vs oracle"
+
+
 def _compare(mix, alone, served, what):
     for i, (item, (ak, av), (sk, sv)) in enumerate(zip(mix, alone, served)):
         op, tag, vtag, keys, vals, k, hasnan, desc = item
         w = f"{what} request {i}: {op}<{tag}{',' + vtag if vtag else ''}> n={keys.size} k={k} " \
             f"hasnan={hasnan} desc={desc}"
-        if op in ("qsort", "keyvalue_qsort"):
+        _oracle_check(item, sk, sv, w)
+        if op == "qsort":
             assert np.array_equal(T.bits(sk), T.bits(ak)), w + ": keys differ"
-            if op == "keyvalue_qsort":
-                assert np.array_equal(T.bits(sv), T.bits(av)), w + ": values differ"
+        elif op == "keyvalue_qsort":
+            # the same keys bit for bit; payloads as a multiset per run of
+            # equal keys (a coalesced launch may order a run differently)
+            assert np.array_equal(T.bits(sk), T.bits(ak)), w + ": keys differ"
+            T.assert_kv_equal(sk, sv, ak, av, tag, w + " vs alone")
         elif op == "qselect":
             assert np.array_equal(np.sort(T.bits(sk)), np.sort(T.bits(keys))), w + ": not a permutation"
             cs, ca = T.vclass(sk, tag), T.vclass(ak, tag)
@@ -168,3 +197,86 @@ def test_serving_poisson_open_loop(xss):
     assert (lat >= 0).all()
     served = [(d.numpy(), v.numpy() if v is not None else None) for d, v in arrays]
     _compare(mix, alone, served, "poisson")
+
+
+def _c5_trace(count, seed):
+    """Requests shaped like the served workload: (op, tag, vtag, keys, vals, k)
+    with keys and values as torch tensors on the GPU."""
+    import torch
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    out = []
+    for _ in range(count):
+        u = rng.random()
+        if u < 0.7:
+            n = int(np.exp(rng.uniform(np.log(1 << 10), np.log(16 << 20))))
+            keys = (torch.rand(n, device="cuda", generator=g) * 2 - 1).view(torch.int32)
+            out.append(("qsort", "f32", None, keys, None, 0))
+        elif u < 0.9:
+            n = int(np.exp(rng.uniform(np.log(1 << 10), np.log(4 << 20))))
+            z = rng.zipf(1.1, n).astype(np.uint64)
+            z[z >= (1 << 32)] = np.uint64((1 << 32) - 1)
+            keys = torch.from_numpy(z.view(np.int64)).cuda()
+            vals = torch.arange(n, dtype=torch.int64, device="cuda")
+            out.append(("keyvalue_qsort", "u64", "u64", keys, vals, 0))
+        else:
+            n = int(np.exp(rng.uniform(np.log(1 << 10), np.log(16 << 20))))
+            keys = torch.randint(-2 ** 31, 2 ** 31, (n,), dtype=torch.int32, device="cuda", generator=g)
+            op = "qselect" if rng.random() < 0.5 else "partial_qsort"
+            k = int(rng.integers(1, n + 1))
+            out.append((op, "u32", None, keys, None, k - 1 if op == "qselect" else k))
+    return out
+
+
+def test_serving_c5_trace(xss):
+    """An open-loop Poisson trace at 10k req/s of the served workload's mix;
+    every result checked: against the oracle up to 2^20 keys, through the
+    full-size properties above."""
+    import torch
+    import xss_devcheck as D
+    trace = _c5_trace(120, 505)
+    rng = np.random.default_rng(6)
+    arrivals = np.cumsum(rng.exponential(100.0, len(trace)))
+    work, reqs = [], []
+    for (op, tag, vtag, keys, vals, k), t in zip(trace, arrivals):
+        gk = keys.clone()
+        gv = vals.clone() if vals is not None else None
+        dk = xss.DArray(tag, gk.numel(), gk.view(torch.uint8))
+        dv = xss.DArray(vtag, gv.numel(), gv.view(torch.uint8)) if gv is not None else None
+        work.append((gk, gv))
+        reqs.append(xss.make_request(op, dk, dv, k, False, False, arrival_us=t))
+    torch.cuda.synchronize()
+    with xss.Server(budget_bytes=64 << 30, streams=4) as sv:
+        done = sv.run_trace(reqs)
+        stats = sv.stats()
+    assert stats["completed"] == len(trace)
+    assert all(r.latency_us >= 0 for r in done)
+    O = T.ORACLE()
+    for i, ((op, tag, vtag, keys, vals, k), (gk, gv)) in enumerate(zip(trace, work)):
+        n = keys.numel()
+        w = f"c5 request {i}: {op}<{tag}> n={n} k={k}"
+        if n <= (1 << 20):
+            hk = keys.cpu().numpy().view(T.TAGS[tag])
+            sk = gk.cpu().numpy().view(T.TAGS[tag])
+            if op == "keyvalue_qsort":
+                hv = vals.cpu().numpy().view(np.uint64)
+                _oracle_check((op, tag, vtag, hk, hv, k, False, False), sk, gv.cpu().numpy().view(np.uint64), w)
+            else:
+                _oracle_check((op, tag, vtag, hk, None, k, False, False), sk, None, w)
+            continue
+        if op == "qsort":
+            D.check_sort(gk, keys, tag, False, False, True, w)
+        elif op == "keyvalue_qsort":
+            D.check_gather(gk, keys, gv, w)
+            D.check_sort(gk, keys, tag, False, False, False, w)
+        else:
+            srt = torch.sort(D.order_key(keys, tag)).values
+            og = D.order_key(gk, tag)
+            assert torch.equal(torch.sort(og).values, srt), w + ": not a permutation"
+            if op == "qselect":
+                assert og[k] == srt[k], w + ": k-th value"
+                assert bool((og[:k] <= srt[k]).all()) and bool((og[k + 1:] >= srt[k]).all()), w + ": partition"
+            else:
+                assert torch.equal(og[:k], srt[:k]), w + ": sorted prefix"
+                assert bool((og[k:] >= srt[k - 1]).all()), w + ": partition"

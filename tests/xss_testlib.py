@@ -340,3 +340,35 @@ def assert_selected(got, inp, exp_sorted, k, tag, what="", zeros_as_values=False
     assert cg[k] == ce[k], f"{what}: k-th element {got[k]} != {exp_sorted[k]}"
     assert np.array_equal(np.sort(cg[:k]), np.sort(ce[:k])), f"{what}: left partition differs"
     assert np.array_equal(np.sort(cg[k + 1:]), np.sort(ce[k + 1:])), f"{what}: right partition differs"
+
+
+def assert_pairs_preserved(gk, gv, k, v, what=""):
+    """Every (key, value) pair of the output is an input pair, each used once
+    (bit patterns)."""
+    pin = np.lexsort((bits(v), bits(k)))
+    pout = np.lexsort((bits(gv), bits(gk)))
+    assert np.array_equal(bits(k)[pin], bits(gk)[pout]) and np.array_equal(bits(v)[pin], bits(gv)[pout]), \
+        f"{what}: (key, value) pairs are not the input's"
+
+
+def assert_kv_partial(gk, gv, k, ink, inv, ek, ev, tag, what=""):
+    """keyvalue_partial_sort(k) result (gk, gv) of input (ink, inv), against
+    the oracle's full key-value sort (ek, ev), with the reference's checker
+    rules (tests/test-keyvalue.cpp:102-171): keys of the first k as the
+    sorted prefix; the values of every run of equal keys that ends inside the
+    prefix as a multiset; the run cut at k holds values of that run's keys;
+    and pairs are never split."""
+    assert_pairs_preserved(gk, gv, ink, inv, what)
+    assert_prefix_equal(gk, ek, k, tag, what)
+    if k == 0:
+        return
+    ce = vclass(ek, tag)
+    kfull = k
+    if k < ek.size and ce[k] == ce[k - 1]:  # the run at k-1 goes on past the prefix
+        kfull = int(np.argmax(ce == ce[k - 1]))
+    assert_kv_equal(gk[:kfull], gv[:kfull], ek[:kfull], ev[:kfull], tag, what + " runs inside the prefix")
+    if kfull < k:  # values of the cut run: a sub-multiset of that run's values
+        pv, pc = np.unique(bits(ev)[ce == ce[k - 1]], return_counts=True)
+        gvv, gc = np.unique(bits(gv)[kfull:k], return_counts=True)
+        j = np.minimum(np.searchsorted(pv, gvv), pv.size - 1)
+        assert bool(((pv[j] == gvv) & (pc[j] >= gc)).all()), f"{what}: a value in the cut run is not from that run"

@@ -69,12 +69,8 @@ XSS_API int xss_gpu_timing_enable(int on);
 /* Runtime tuning switches (no counterpart in the reference):
  *   "hybrid_min_n"  qsort / keyvalue_qsort / argsort of 4- and 8-byte
  *                   keys sort arrays of at least this many keys MSD-first
- *                   (two passes over HBM, then every bucket of the top 16
- *                   bits in LDS); default 2^25.
- *   "hybrid_check"  1 (default): before an MSD-first sort, a sample of 4096
- *                   keys (one small kernel and one wait for the stream; not
- *                   while the stream is being captured) sends skewed or
- *                   duplicate-heavy data to the LSD path instead; 0: no check.
+ *                   (two passes over HBM, then every bucket of the two
+ *                   most significant varying digits in LDS); default 2^25.
  *   "count_min_n"   qsort of 16-bit keys counts keys per value (a counting
  *                   sort: one read, one write) for arrays of at least this
  *                   many keys; default 2^23. */

@@ -1,6 +1,7 @@
 """GPU parity of the MSD-first sort path (two passes over HBM, then every
-bucket of the top 16 bits in LDS; buckets too large for LDS take segmented
-passes) for qsort, keyvalue_qsort and argsort.  By default only arrays of
+bucket of the two most significant varying digits in LDS; buckets too large
+for LDS take segmented passes, buckets of equal keys none) for qsort,
+keyvalue_qsort and argsort.  By default only arrays of
 2^25+ keys take it; here the switch xss_gpu_set_tuning("hybrid_min_n", 2)
 sends every 4- and 8-byte sort through it, so small seeded arrays exercise
 empty, single-key, LDS-sized and large buckets (duplicates, constants, NaN
@@ -26,12 +27,9 @@ DISTS = ["random", "dups", "constant", "sorted", "reverse", "smallrange", "signe
 def hybrid_everywhere(xss):
     L = xss.lib()
     L.xss_gpu_set_tuning.argtypes = [ctypes.c_char_p, ctypes.c_uint64]
-    # every size, and no sampled fallback to the LSD path
-    assert L.xss_gpu_set_tuning(b"hybrid_min_n", 2) == 0
-    assert L.xss_gpu_set_tuning(b"hybrid_check", 0) == 0
+    assert L.xss_gpu_set_tuning(b"hybrid_min_n", 2) == 0  # every size
     yield xss
     assert L.xss_gpu_set_tuning(b"hybrid_min_n", 1 << 25) == 0
-    assert L.xss_gpu_set_tuning(b"hybrid_check", 1) == 0
 
 
 @pytest.mark.parametrize("desc", [False, True])
@@ -88,3 +86,44 @@ def test_hybrid_argsort(hybrid_everywhere, tag, desc):
         T.assert_argsort(idx, a, exp, tag, f"hybrid argsort<{tag}> {dist} n={n} desc={desc}")
         assert np.array_equal(T.bits(d.numpy()), T.bits(a)), "argsort must leave the keys untouched"
 
+
+
+def narrow(tag, n, bits_, seed):
+    """Keys whose order keys differ only in their low `bits_` bits: the device
+    plan then starts the MSD passes lower (64-bit keys below 2^32 sort like
+    32-bit keys)."""
+    rng = np.random.default_rng(seed)
+    nb = np.dtype(T.TAGS[tag]).itemsize * 8
+    u = T.UINT[nb // 8]
+    # non-negative values below 2^bits_ (for signed types the order key is
+    # bits ^ sign, whose top bits are then constant too)
+    x = rng.integers(0, 2 ** min(bits_, nb - 1), n, dtype=np.uint64).astype(u)
+    return x.view(T.TAGS[tag])
+
+
+@pytest.mark.parametrize("tag,bits_", [("u64", 32), ("u64", 40), ("u64", 24), ("i64", 33), ("u64", 12),
+                                       ("u64", 7), ("u32", 24), ("u32", 16), ("i32", 9), ("u32", 0)])
+@pytest.mark.parametrize("op", ["qsort", "kv", "argsort"])
+def test_hybrid_narrow_keys(hybrid_everywhere, tag, bits_, op):
+    """Keys with constant top digits: the plan made on the device skips them
+    (in pairs), including the case with no low digit left and with a single
+    value (bits_ = 0)."""
+    xss = hybrid_everywhere
+    O = T.ORACLE()
+    for seed, n in enumerate([3, 5000, 70001, 1 << 20]):
+        a = narrow(tag, n, bits_, 100 + seed)
+        what = f"narrow {op}<{tag}> bits={bits_} n={n}"
+        for desc in (False, True):
+            d = xss.DArray.from_numpy(a)
+            if op == "qsort":
+                xss.qsort(d, False, desc)
+                T.assert_sorted_equal(d.numpy(), O.qsort(a, tag, False, desc), tag, what)
+            elif op == "kv":
+                v = np.arange(n, dtype=np.uint64)
+                dv = xss.DArray.from_numpy(v)
+                xss.keyvalue_qsort(d, dv, False, desc)
+                ek, ev = O.keyvalue_qsort(a, v, tag, "u64", False, desc)
+                T.assert_kv_equal(d.numpy(), dv.numpy(), ek, ev, tag, what)
+            else:
+                idx = xss.argsort(d, False, desc).numpy()
+                T.assert_argsort(idx, a, a[O.argsort(a, tag, False, desc).astype(np.int64)], tag, what)

@@ -1,30 +1,36 @@
-// xss_hybrid.hpp -- MSD-first sort of one large keys-only array.
+// xss_hybrid.hpp -- MSD-first sort of one large array (keys only, key-value,
+// or argsort).
 //
 // An LSD sort of 4-byte keys streams the array through HBM four times (each
 // pass: a histogram read, then a read and a write).  For large arrays this
 // path does two passes and then finishes every bucket on chip:
 //
-//   pass A   LSD pass on the top digit over the whole array (keys -> scratch)
+//   pass A   LSD pass on the top digit over the whole array (keys -> scratch);
+//            its histogram read also takes the AND / OR of all keys, and a
+//            plan made on the device moves passes A and B down to the
+//            highest digits on which keys differ (keys below 2^32 in a
+//            64-bit type sort like 32-bit keys)
 //   table    the 256 top-digit buckets are cut into ranges (built on the
 //            device from pass A's digit totals)
-//   pass B   segmented pass on the second digit inside every top bucket
+//   pass B   segmented pass on the next digit inside every top bucket
 //            (scratch -> keys); its per-bucket digit totals are the exact
-//            sizes of the 65536 buckets of the top two digits
+//            sizes of the 65536 buckets of the two MSD digits
 //   buckets  bucket starts; buckets larger than one workgroup's LDS are listed
 //   local    one workgroup per bucket sorts it by the remaining low digits in
 //            LDS (an 8-bit LSD radix per digit, stable) and writes it back
 //   large    listed buckets (skewed or duplicate-heavy data) get segmented
-//            LSD passes on the low digits (keys -> scratch -> keys)
+//            LSD passes on the low digits (keys -> scratch -> keys); a
+//            bucket whose keys are all equal is left in place
 //
-// Everything is enqueued on the stream without a host round trip (the range
-// tables are built by kernels and passes over them are launched at the
-// tables' capacity, surplus workgroups exiting), so the call stays
-// asynchronous and capturable.  Keys-only sorts need no stability, but every
-// step is stable anyway.
+// Everything is enqueued on the stream without a host round trip: the range
+// tables and the plan are built by kernels, passes over the tables hand
+// their ranges out from counters to a bounded grid, and passes the plan
+// does not need exit at once.  The call stays asynchronous and capturable.
+// Passes whose input order does not matter rank without keeping it, except
+// for argsort, whose every pass is stable.
 #pragma once
 #include <atomic>
 #include <type_traits>
-#include <vector>
 
 #include "xss_count.hpp"
 #include "xss_sort.hpp"
@@ -73,7 +79,62 @@ struct HybArgs {
     HybTable td;                // ranges of the low passes (segments: large buckets)
     uint64_t rk_b;              // keys per range
     uint64_t rk_d;
+    uint32_t *wctr;             // work-stealing counters of the segmented passes (cleared by top_ranges)
+    uint32_t nwctr;
+    uint64_t *seg_andor;        // [lcap][2] AND / OR of each large bucket's raw key bits
+    const uint32_t *plan;       // the device plan (hyb_plan_kernel)
+    uint32_t *lrfirst;          // [lcap + 1] first range of each large bucket (large_ranges)
 };
+
+// The sort's plan, made on the device from the AND / OR of all order keys
+// (pass A's histogram read): digits above the highest one on which keys
+// differ are skipped in pairs, so that an even count of low digits remains
+// and the keys end in the caller's buffer.  plan[0] = shift of pass A's
+// digit dA, plan[1] = shift of pass B's digit dA - 1, plan[2] = the low
+// digits (dA - 1) sorted in LDS or by the large-bucket passes, plan[3] = 1
+// if dA is not the top digit (pass A's histogram is then taken again),
+// plan[4] = 1 if no low digit is left (dA = 1).  Keys that differ only in
+// digit 0 (or not at all) take dA = 1.
+template <int P>
+__global__ __launch_bounds__(256) void hyb_plan_kernel(const uint64_t *slots, uint32_t nslots, uint64_t *andor,
+                                                       uint32_t *plan)
+{
+    fold_andor(slots, nslots, andor);  // pass A's per-workgroup AND / OR
+    if (threadIdx.x != 0) return;
+    const uint64_t diff = andor[0] ^ andor[1];
+    int t = -1;
+    for (int d = P - 1; d >= 0; --d)
+        if ((diff >> (8 * d)) & 0xffu) {
+            t = d;
+            break;
+        }
+    int dA = t < 1 ? 1 : ((P - 1 - t) % 2 == 0 ? t : t + 1);
+    plan[0] = uint32_t(8 * dA);
+    plan[1] = uint32_t(8 * (dA - 1));
+    plan[2] = uint32_t(dA - 1);
+    plan[3] = dA != P - 1 ? 1u : 0u;
+    plan[4] = dA == 1 ? 1u : 0u;
+}
+
+// A plan without low digits (plan[4]): after passes A and B every bucket
+// holds equal keys, so the sort is done except for what the last pass
+// writes: argsort's indices (from the payload, in order) and qsort(hasnan)'s
+// canonical NaNs.  Grid-stride over the array; exits unless plan[4] is set.
+template <class KD, int PL>
+__global__ __launch_bounds__(256) void hyb_finish_kernel(const uint32_t *plan, void *keys_v, const void *vals_v,
+                                                         uint64_t *idx_out, uint64_t n, uint32_t canon)
+{
+    using U = typename KD::U;
+    if (gate_closed(plan + 4, 0)) return;
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256) {
+        if constexpr (PL == PL_IDX) {
+            idx_out[i] = uint64_t(static_cast<const uint32_t *>(vals_v)[i]);
+        } else if constexpr (KD::KIND == Kind::Float) {
+            U *k = static_cast<U *>(keys_v);
+            if (canon && KD::is_nan(k[i])) k[i] = KD::QNAN;
+        }
+    }
+}
 
 // ranges of pass B from pass A's digit totals (one workgroup)
 template <int = 0>
@@ -104,21 +165,35 @@ __global__ __launch_bounds__(256) void hyb_top_ranges_kernel(HybArgs a)
     }
     const uint64_t start = p1 + i1 - size;
     const uint32_t r0 = p2 + i2 - nrs;
+    __shared__ uint32_t s_r0[257];
+    __shared__ uint64_t s_start[256];
     a.seg_off[s] = start;
+    s_r0[s] = r0;
+    s_start[s] = start;
     if (s == 255) {
         a.seg_off[256] = start + size;
         *a.tb.nr = r0 + nrs;
         *a.nlarge = 0;
+        s_r0[256] = r0 + nrs;
     }
-    for (uint32_t j = 0; j < nrs; ++j) {
-        const uint32_t r = r0 + j;
-        if (r >= a.tb.cap) break;  // cannot happen: cap = ceil(n / rk) + 256
-        const uint64_t b = start + uint64_t(j) * a.rk_b;
+    if (s < a.nwctr) a.wctr[s] = 0;
+    __syncthreads();
+    // every range by its own thread (one bucket may hold nearly every key)
+    const uint32_t total = min(s_r0[256], a.tb.cap);  // cap = ceil(n / rk) + 256: never below the count
+    for (uint32_t r = s; r < total; r += 256) {
+        uint32_t lo = 0, hi = 256;  // the top bucket with s_r0[lo] <= r < s_r0[lo + 1]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (s_r0[mid] <= r) lo = mid;
+            else hi = mid;
+        }
+        const uint64_t st = s_start[lo], end = lo == 255 ? st + a.top_total[255] : s_start[lo + 1];
+        const uint64_t b = st + uint64_t(r - s_r0[lo]) * a.rk_b;
         a.tb.begin[r] = b;
-        a.tb.end[r] = min(start + size, b + a.rk_b);
-        a.tb.sbase[r] = start;
-        a.tb.bseg[r] = s;
-        a.tb.head[r] = j == 0 ? 1u : 0u;
+        a.tb.end[r] = min(end, b + a.rk_b);
+        a.tb.sbase[r] = st;
+        a.tb.bseg[r] = lo;
+        a.tb.head[r] = b == st ? 1u : 0u;
     }
 }
 
@@ -145,6 +220,12 @@ __global__ __launch_bounds__(256) void hyb_buckets_kernel(HybArgs a)
     const uint32_t start = uint32_t(s0) + pre + incl - size;
     const uint32_t b = s * 256 + d;
     a.bstart[b] = start;
+    // a plan without low digits (plan[4]) leaves every bucket as it is:
+    // marked larger than any LDS capacity, and listed nowhere
+    if (a.plan[4]) {
+        a.bsize[b] = ~0u;
+        return;
+    }
     a.bsize[b] = size;
     if (size > a.cap) {
         const uint32_t i = atomicAdd(a.nlarge, 1u);
@@ -155,11 +236,14 @@ __global__ __launch_bounds__(256) void hyb_buckets_kernel(HybArgs a)
     }
 }
 
-// ranges of the low passes over the large buckets (one workgroup)
+// ranges of the low passes over the large buckets (one workgroup): the
+// first range of every bucket by a scan, then every range written by its own
+// thread (a bucket of millions of keys has thousands of ranges)
 template <int = 0>
 __global__ __launch_bounds__(1024) void hyb_large_ranges_kernel(HybArgs a)
 {
     __shared__ uint32_t wsum[16];
+    __shared__ uint32_t s_total;
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint32_t nl = min(*a.nlarge, a.lcap);
     const uint32_t per = (nl + 1023) / 1024;
@@ -176,17 +260,34 @@ __global__ __launch_bounds__(1024) void hyb_large_ranges_kernel(HybArgs a)
     __syncthreads();
     uint32_t r = incl - cnt;
     for (uint32_t k = 0; k < w; ++k) r += wsum[k];
-    if (t == 1023) *a.td.nr = min(a.td.cap, r + cnt);
     for (uint32_t i = c0; i < c1; ++i) {
-        const uint64_t st = a.lstart[i], sz = a.lsize[i];
-        for (uint64_t x = st; x < st + sz; x += a.rk_d, ++r) {
-            if (r >= a.td.cap) return;  // cannot happen: cap = ceil(n / rk) + lcap
-            a.td.begin[r] = x;
-            a.td.end[r] = min(st + sz, x + a.rk_d);
-            a.td.sbase[r] = st;
-            a.td.bseg[r] = i;
-            a.td.head[r] = x == st ? 1u : 0u;
+        a.seg_andor[2 * size_t(i)] = ~0ull;
+        a.seg_andor[2 * size_t(i) + 1] = 0ull;
+        a.lrfirst[i] = r;
+        r += uint32_t((a.lsize[i] + a.rk_d - 1) / a.rk_d);
+    }
+    if (t == 1023) {
+        const uint32_t total = min(a.td.cap, r);  // cap = ceil(n / rk) + lcap: never below r
+        *a.td.nr = total;
+        a.lrfirst[nl] = r;
+        s_total = total;
+    }
+    __syncthreads();
+    const uint32_t total = s_total;
+    for (uint32_t q = t; q < total; q += 1024) {
+        uint32_t lo = 0, hi = nl;  // the bucket i with lrfirst[i] <= q < lrfirst[i + 1]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (a.lrfirst[mid] <= q) lo = mid;
+            else hi = mid;
         }
+        const uint64_t st = a.lstart[lo], sz = a.lsize[lo];
+        const uint64_t x = st + uint64_t(q - a.lrfirst[lo]) * a.rk_d;
+        a.td.begin[q] = x;
+        a.td.end[q] = min(st + sz, x + a.rk_d);
+        a.td.sbase[q] = st;
+        a.td.bseg[q] = lo;
+        a.td.head[q] = x == st ? 1u : 0u;
     }
 }
 
@@ -218,7 +319,19 @@ struct HybShape {
     static constexpr bool HAS_SMALL = BYTES > 4;
     static constexpr bool HAS_MID = BYTES <= 8;
 };
-constexpr uint32_t HYB_LARGE_GRID = 1024;  // workgroups of each large-bucket pass
+// Grids of the segmented passes, whose ranges are handed out by a counter
+// (work stealing): a scatter workgroup holds a CU (1024 threads), so one per
+// CU; histogram workgroups (256 threads) four per CU.  A pass with little
+// or no work (the large-bucket passes of spread data) then costs little.
+#ifndef XSS_HYB_DOWN_GRID
+#define XSS_HYB_DOWN_GRID 256
+#endif
+#ifndef XSS_HYB_UP_GRID
+#define XSS_HYB_UP_GRID 1024
+#endif
+// control words of the sort: plan[8], AND / OR [2 x u64], work counters
+constexpr size_t HYB_CTL_BYTES = 32 + 16 + 64 * 4;
+constexpr uint32_t HYB_NWCTR = 64;
 
 // Tile shape of the segmented passes of this sort: the batch form's
 // (SegShape), except 4-byte keys, which take 1024 x 16 tiles as the
@@ -257,7 +370,7 @@ constexpr uint32_t HYB_SMALL_MARGIN = 384;
 template <class KD, int PL, bool NL, int SIZE>
 __global__ __launch_bounds__(HYB_BLOCK) __attribute__((amdgpu_waves_per_eu(HybShape<KD, PL, SIZE>::WAVES)))
 void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_v, void *vals_v, uint64_t *idx_out,
-                      uint64_t flip64, uint32_t canon)
+                      uint64_t flip64, uint32_t canon, const uint32_t *plan)
 {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
@@ -272,7 +385,7 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
     __shared__ uint32_t s_wsum[WAVES];
     const uint32_t b = blockIdx.x;
     const uint32_t m = bsize[b];
-    if (m > CAP) return;  // a large bucket: the low segmented passes sort it
+    if (m > CAP) return;  // a large bucket: the low segmented passes sort it (or a plan with no low digit)
     const uint32_t st = bstart[b];
     U *keys = static_cast<U *>(keys_v) + st;
     V *vals = HAS_PL ? static_cast<V *>(vals_v) + st : nullptr;
@@ -400,8 +513,11 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
         pass(0, true);
         pass(8, false);
     } else {
+        // low digits to sort (plan[2]): P - 2, or fewer when the top digits
+        // are the same in every key
+        const int npass = int(__builtin_amdgcn_readfirstlane(plan[2]));
 #pragma unroll 1
-        for (int p = 0; p < P - 2; ++p) pass(8 * p, p == 0);
+        for (int p = 0; p < npass; ++p) pass(8 * p, p == 0);
     }
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
@@ -456,7 +572,8 @@ struct HybPlan {
     {
         return data_bytes(n) + align256(256 * 256 * 4) + align256(257 * 8) + 2 * align256(HYB_BUCKETS * 4) +
                align256(16) + table_bytes(cap_b(n)) + 2 * align256(size_t(lcap(n)) * 4) + table_bytes(cap_d(n)) +
-               2 * align256(size_t(256) * cap_d(n) * 4) + align256(size_t(lcap(n)) * 256 * 4);
+               2 * align256(size_t(256) * cap_d(n) * 4) + align256(size_t(lcap(n)) * 256 * 4) +
+               align256(HYB_CTL_BYTES) + align256(size_t(lcap(n)) * 16) + align256((size_t(lcap(n)) + 1) * 4);
     }
 };
 
@@ -516,6 +633,14 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     uint32_t *rsum_d = reinterpret_cast<uint32_t *>(take(size_t(256) * cap_d * 4));
     uint32_t *rpre_d = reinterpret_cast<uint32_t *>(take(size_t(256) * cap_d * 4));
     uint32_t *segtot_d = reinterpret_cast<uint32_t *>(take(size_t(lcap) * 256 * 4));
+    char *ctl = take(HYB_CTL_BYTES);
+    uint32_t *plan = reinterpret_cast<uint32_t *>(ctl);
+    uint64_t *andor = reinterpret_cast<uint64_t *>(ctl + 32);
+    h.wctr = reinterpret_cast<uint32_t *>(ctl + 48);
+    h.nwctr = HYB_NWCTR;
+    h.seg_andor = reinterpret_cast<uint64_t *>(take(size_t(lcap) * 16));
+    h.lrfirst = reinterpret_cast<uint32_t *>(take((size_t(lcap) + 1) * 4));
+    h.plan = plan;
     h.top_total = ws->total();
     h.segtot = segtot;
     h.nlarge = ctr;
@@ -524,10 +649,51 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     h.rk_b = HP::rk_b(n);
     h.rk_d = HP::rk_d(n);
 
-    // pass A: top digit over the whole array (keys -> Y; argsort: indices generated)
+    // pass A: top digit over the whole array (keys -> Y; argsort: indices
+    // generated).  Its histogram read also takes the AND / OR of every order
+    // key, from which the plan kernel picks the digit pass A really sorts
+    // by (the top one unless all keys share it); another histogram is taken
+    // only in that case.
     {
-        const int rc = enqueue_lsd_pass<KD, PL>(ws, keys, vals, yk, yv, nullptr, n, 8 * (P - 1), flip, nan_last,
-                                                true, false, false, s, nullptr, 0, false, nullptr, true);
+        using Plan = SortPlan<KD, PL>;
+        const uint32_t rtiles = Plan::range_tiles(n), ranges = Plan::ranges(n);
+        UpArgs u{};
+        u.keys = keys; u.n = nn; u.range_keys = uint32_t(rtiles * Plan::tile(n)); u.shift = uint32_t(8 * (P - 1));
+        u.nan_last = nan_last; u.flip = flip; u.rsum = ws->rsum(); u.ranges = ranges; u.andor = andor; u.pass = 0;
+        u.andor_slots = ws->andor_slots();
+        {
+            KernelTimer t("upsweep", s);
+            hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
+        }
+        XSS_HIP_CHECK(hipGetLastError());
+        {
+            KernelTimer t("hyb_tables", s);
+            hipLaunchKernelGGL(hyb_plan_kernel<P>, dim3(1), dim3(256), 0, s, ws->andor_slots(), ranges, andor, plan);
+        }
+        XSS_HIP_CHECK(hipGetLastError());
+        UpArgs u2 = u;  // the histogram of plan[0]'s digit, if it is not the top one
+        u2.andor = nullptr; u2.shift_dev = plan; u2.gate = plan + 3; u2.gate_min = 0; u2.nr_static = ranges;
+        {
+            KernelTimer t("upsweep", s);
+            hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(std::min<uint32_t>(ranges, 512)), dim3(UP_BLOCK), 0, s,
+                               u2);
+        }
+        XSS_HIP_CHECK(hipGetLastError());
+        ScanArgs sc{};
+        sc.rsum = ws->rsum(); sc.rpre = ws->rpre(); sc.total = ws->total(); sc.ranges = ranges;
+        {
+            KernelTimer t("range_scan", s);
+            hipLaunchKernelGGL(range_scan_kernel<0>, dim3(256), dim3(SCAN_BLOCK), 0, s, sc);
+        }
+        XSS_HIP_CHECK(hipGetLastError());
+        PassArgs a{};
+        a.keys_in = keys; a.vals_in = vals; a.keys_out = yk; a.vals_out = yv; a.idx_out = nullptr;
+        a.n = nn; a.shift = uint32_t(8 * (P - 1)); a.shift_dev = plan; a.flip = flip; a.nan_last = nan_last;
+        a.range_tiles = rtiles; a.ranges = ranges; a.rpre = ws->rpre(); a.total = ws->total();
+        a.first = true; a.last = false; a.canon_nan = false;
+        const int rc = Plan::small_tiles(n)
+                           ? launch_downsweep<KD, PL, SmallPassShape<KD, PL>>(a, ranges, nan_last, true, s)
+                           : launch_downsweep<KD, PL, PassShape<KD, PL>>(a, ranges, nan_last, true, s);
         if (rc) return rc;
     }
     {
@@ -535,22 +701,39 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
         hipLaunchKernelGGL(hyb_top_ranges_kernel<0>, dim3(1), dim3(256), 0, s, h);
     }
     XSS_HIP_CHECK(hipGetLastError());
+    uint32_t wnext = 0;  // next free work-stealing counter
     // one segmented pass over a device-built table (src -> dst)
     auto seg_pass = [&](const char *const names[3], const HybTable &t, uint32_t *rsum, uint32_t *rpre,
-                        uint32_t *stot, int shift, const void *sk, const void *sv, void *dk, void *dv,
-                        bool last, bool unstable, uint32_t max_grid) -> int {
-        // the kernels walk the table's ranges with a bounded grid: a table
-        // that turns out (nearly) empty costs few workgroups
-        const uint32_t grid = std::min<uint32_t>(t.cap, max_grid);
+                        uint32_t *stot, int shift, const uint32_t *shift_dev, const uint32_t *gate,
+                        uint32_t gate_min, const void *sk, const void *sv, void *dk, void *dv, bool last,
+                        bool unstable, bool large, bool first_large) -> int {
+        // both kernels walk the table's ranges from a counter with a bounded
+        // grid: a table that turns out (nearly) empty costs few workgroups
+        // the large-bucket passes (usually near empty) take their ranges from
+        // counters with small grids; pass B (always full) one range per
+        // workgroup, as a counter's round trip per range costs it time
+        uint32_t *up_ctr = large ? h.wctr + wnext++ : nullptr;
+        uint32_t *down_ctr = large ? h.wctr + wnext++ : nullptr;
         UpArgs u{};
-        u.keys = sk; u.n = nn; u.shift = uint32_t(shift); u.nan_last = nan_last; u.flip = flip;
-        u.rsum = rsum; u.ranges = t.cap; u.range_begin = t.begin; u.range_end = t.end; u.nr_dev = t.nr;
+        u.keys = sk; u.n = nn; u.shift = uint32_t(shift); u.shift_dev = shift_dev; u.nan_last = nan_last;
+        u.flip = flip; u.rsum = rsum; u.ranges = t.cap; u.range_begin = t.begin; u.range_end = t.end;
+        u.nr_dev = t.nr; u.gate = gate; u.gate_min = gate_min; u.wctr = up_ctr; u.rbseg = t.bseg;
+        if (large) {
+            if (first_large) u.seg_andor_out = h.seg_andor;
+            else u.seg_andor_in = h.seg_andor;
+            u.seg_canon = canon_nan;
+        }
         {
             KernelTimer kt(names[0], s);
-            hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(grid), dim3(UP_BLOCK), 0, s, u);
+            if (large)
+                hipLaunchKernelGGL((upsweep_kernel<KD, true>), dim3(std::min<uint32_t>(t.cap, XSS_HYB_UP_GRID)),
+                                   dim3(UP_BLOCK), 0, s, u);
+            else
+                hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(std::min<uint32_t>(t.cap, MAX_RANGES)), dim3(UP_BLOCK),
+                                   0, s, u);
         }
         XSS_HIP_CHECK(hipGetLastError());
-        SegScanArgs sc{rsum, rpre, stot, t.head, t.bseg, 0, t.cap, t.nr};
+        SegScanArgs sc{rsum, rpre, stot, t.head, t.bseg, 0, t.cap, t.nr, gate, gate_min, down_ctr};
         {
             KernelTimer kt(names[1], s);
             hipLaunchKernelGGL(seg_scan_kernel<0>, dim3(256), dim3(256), 0, s, sc);
@@ -558,35 +741,52 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
         XSS_HIP_CHECK(hipGetLastError());
         PassArgs a{};
         a.keys_in = sk; a.vals_in = sv; a.keys_out = dk; a.vals_out = dv; a.idx_out = idx_out; a.n = nn;
-        a.shift = uint32_t(shift); a.flip = flip; a.nan_last = nan_last; a.first = false; a.last = last;
+        a.shift = uint32_t(shift); a.shift_dev = shift_dev; a.gate = gate; a.gate_min = gate_min; a.flip = flip;
+        a.nan_last = nan_last; a.first = false; a.last = last;
         a.canon_nan = last && canon_nan;
-        SegRanges r{t.begin, t.end, t.sbase, t.bseg, rpre, stot, 0, t.cap, t.nr};
+        SegRanges r{t.begin, t.end, t.sbase, t.bseg, rpre, stot, 0, t.cap, t.nr, down_ctr};
+        if (large) {
+            r.seg_andor = h.seg_andor;
+            r.seg_canon = canon_nan;
+            r.home_vals = xv;
+            a.last_gate = plan + 2;
+            a.last_q = gate_min;
+            a.canon_if_last = canon_nan;
+        }
+        const uint32_t grid = std::min<uint32_t>(t.cap, large ? XSS_HYB_DOWN_GRID * (1024 / SS::BLOCK) : MAX_RANGES);
         KernelTimer kt(names[2], s);
         // pass B's input order does not matter (it is an MSD pass inside
         // each top bucket), so it ranks without keeping it; argsort keeps
         // every pass stable (ties in input order, the identity for sorted
         // input)
         constexpr bool UN = PL != PL_IDX;
-        if (unstable && nan_last)
-            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, SS::BLOCK, SS::ITEMS, true, UN>), dim3(grid),
-                               dim3(SS::BLOCK), 0, s, a, r);
-        else if (unstable)
-            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, SS::BLOCK, SS::ITEMS, false, UN>), dim3(grid),
-                               dim3(SS::BLOCK), 0, s, a, r);
-        else if (nan_last)
-            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, SS::BLOCK, SS::ITEMS, true>), dim3(grid),
-                               dim3(SS::BLOCK), 0, s, a, r);
-        else
-            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, SS::BLOCK, SS::ITEMS, false>), dim3(grid),
-                               dim3(SS::BLOCK), 0, s, a, r);
+        auto go = [&](auto nl_tag, auto un_tag, auto seg_tag) {
+            hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, SS::BLOCK, SS::ITEMS, decltype(nl_tag)::value,
+                                                     decltype(un_tag)::value, decltype(seg_tag)::value>),
+                               dim3(grid), dim3(SS::BLOCK), 0, s, a, r);
+        };
+        using T_ = std::true_type;
+        using F_ = std::false_type;
+        using U_ = std::integral_constant<bool, UN>;
+        if (large) {
+            if (unstable && nan_last) go(T_{}, U_{}, T_{});
+            else if (unstable) go(F_{}, U_{}, T_{});
+            else if (nan_last) go(T_{}, F_{}, T_{});
+            else go(F_{}, F_{}, T_{});
+        } else {
+            if (unstable && nan_last) go(T_{}, U_{}, F_{});
+            else if (unstable) go(F_{}, U_{}, F_{});
+            else if (nan_last) go(T_{}, F_{}, F_{});
+            else go(F_{}, F_{}, F_{});
+        }
         XSS_HIP_CHECK(hipGetLastError());
         return XSS_OK;
     };
-    // pass B: second digit inside every top bucket (Y -> X)
+    // pass B: the digit below pass A's inside every top bucket (Y -> X)
     static const char *const msd_names[3] = {"hyb_msd_upsweep", "hyb_msd_scan", "hyb_msd_downsweep"};
     static const char *const low_names[3] = {"hyb_large_upsweep", "hyb_large_scan", "hyb_large_downsweep"};
-    int rc = seg_pass(msd_names, h.tb, ws->rsum(), ws->rpre(), segtot, 8 * (P - 2), yk, yv, xk, xv, false, true,
-                      MAX_RANGES);
+    int rc = seg_pass(msd_names, h.tb, ws->rsum(), ws->rpre(), segtot, 8 * (P - 2), plan + 1, nullptr, 0, yk, yv,
+                      xk, xv, false, true, false, false);
     if (rc) return rc;
     {
         KernelTimer t("hyb_tables", s);
@@ -599,10 +799,10 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
             constexpr int SZ = decltype(size_tag)::value;
             if (nan_last)
                 hipLaunchKernelGGL((hyb_local_kernel<KD, PL, true, SZ>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s,
-                                   h.bstart, h.bsize, xk, xv, idx_out, flip, uint32_t(canon_nan));
+                                   h.bstart, h.bsize, xk, xv, idx_out, flip, uint32_t(canon_nan), plan);
             else
                 hipLaunchKernelGGL((hyb_local_kernel<KD, PL, false, SZ>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s,
-                                   h.bstart, h.bsize, xk, xv, idx_out, flip, 0u);
+                                   h.bstart, h.bsize, xk, xv, idx_out, flip, 0u, plan);
         };
         const int sh = HP::shape(n);
         if constexpr (HybShape<KD, PL>::HAS_SMALL) {
@@ -619,20 +819,29 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
         hipLaunchKernelGGL(hyb_large_ranges_kernel<0>, dim3(1), dim3(1024), 0, s, h);
     }
     XSS_HIP_CHECK(hipGetLastError());
-    // low digits of the large buckets (X -> Y -> X ...; P - 2 is even; the
-    // last pass of argsort writes the indices).  Their table is sized for
-    // the worst case (every key in a large bucket) but is usually empty
-    // (spread data), so the passes take HYB_LARGE_GRID persistent
-    // workgroups: 4 per CU still fill the GPU when the table is full, and
-    // an empty table costs a quarter of a 4096-workgroup launch.
+    // low digits of the large buckets (X -> Y -> X ...; the plan's count of
+    // low digits is even; the last pass of argsort writes the indices).
+    // Pass q runs only if the plan has more than q low digits.  The first
+    // pass also takes each large bucket's AND / OR: a bucket of equal keys
+    // (duplicate-heavy data) is then left where it is by every pass.  The
+    // table is sized for the worst case (every key in a large bucket) but
+    // is usually empty (spread data).
     void *sk = xk, *sv = xv, *dk = yk, *dv = yv;
     for (int q = 0; q < P - 2; ++q) {
-        rc = seg_pass(low_names, h.td, rsum_d, rpre_d, segtot_d, 8 * q, sk, sv, dk, dv, q == P - 3, q == 0,
-                      HYB_LARGE_GRID);
+        // the last pass (the plan's last low digit) writes argsort's
+        // indices and canonical NaNs; which one that is is read on the device
+        rc = seg_pass(low_names, h.td, rsum_d, rpre_d, segtot_d, 8 * q, nullptr, plan + 2, uint32_t(q), sk, sv, dk,
+                      dv, false, q == 0, true, q == 0);
         if (rc) return rc;
         std::swap(sk, dk);
         std::swap(sv, dv);
     }
+    if (PL == PL_IDX || canon_nan) {
+        KernelTimer t("hyb_tables", s);
+        hipLaunchKernelGGL((hyb_finish_kernel<KD, PL>), dim3(1024), dim3(256), 0, s, plan, xk, xv, idx_out,
+                           uint64_t(n), uint32_t(canon_nan));
+    }
+    XSS_HIP_CHECK(hipGetLastError());
     return XSS_OK;
 }
 
@@ -640,62 +849,9 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
 
 namespace xss {
 
-// arrays of at least this many keys take the MSD-first path, and whether the
-// sampled spread check runs first (xss_gpu_set_tuning)
+// arrays of at least this many keys take the MSD-first path (xss_gpu_set_tuning)
 std::atomic<uint64_t> &hybrid_min_n();
-std::atomic<uint64_t> &hybrid_check();
 std::atomic<uint64_t> &count_min_n();  // 16-bit qsort: counting sort from this size
-
-constexpr uint32_t HYB_SAMPLES = 4096;
-
-// top 16 bits of the order keys of HYB_SAMPLES evenly spaced keys
-template <class KD>
-__global__ __launch_bounds__(256) void hyb_sample_kernel(const void *keys_v, uint64_t n, uint64_t flip64,
-                                                         uint32_t nan_last, uint16_t *out)
-{
-    using U = typename KD::U;
-    const U *keys = static_cast<const U *>(keys_v);
-    for (uint32_t i = threadIdx.x; i < HYB_SAMPLES; i += 256) {
-        const uint64_t pos = (uint64_t(i) * n) / HYB_SAMPLES;
-        const U o = KD::ord(keys[pos], U(flip64), nan_last != 0);
-        out[i] = uint16_t(o >> (8 * sizeof(U) - 16));
-    }
-}
-
-// Whether the top 16 bits spread the keys over buckets small enough for
-// the LDS sort: in a sample of HYB_SAMPLES keys, samples that share their
-// top 16 bits with another sample point at buckets of >= ~2n/HYB_SAMPLES
-// keys (far above an LDS bucket at these sizes).  Skewed or
-// duplicate-heavy data -- most samples in such buckets -- sorts faster on
-// the LSD path.  Costs one small kernel and one wait for the stream; a
-// stream being captured into a graph cannot wait, and takes the MSD-first
-// path unconditionally.
-template <class KD>
-bool hybrid_spread_ok(Workspace *ws, const void *keys, size_t n, bool nan_last, uint64_t flip, hipStream_t s)
-{
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return true;
-    static thread_local uint16_t *host = nullptr;
-    if (!host && hipHostMalloc(reinterpret_cast<void **>(&host), HYB_SAMPLES * 2) != hipSuccess) {
-        host = nullptr;
-        return true;
-    }
-    uint16_t *dev = reinterpret_cast<uint16_t *>(ws->rpre());  // free until the sort's first scan
-    hipLaunchKernelGGL(hyb_sample_kernel<KD>, dim3(1), dim3(256), 0, s, keys, uint64_t(n), flip,
-                       uint32_t(nan_last), dev);
-    if (hipGetLastError() != hipSuccess) return true;
-    if (hipMemcpyAsync(host, dev, HYB_SAMPLES * 2, hipMemcpyDeviceToHost, s) != hipSuccess) return true;
-    if (hipStreamSynchronize(s) != hipSuccess) return true;
-    static thread_local std::vector<uint8_t> seen(65536, 0);
-    uint32_t shared = 0;
-    for (uint32_t i = 0; i < HYB_SAMPLES; ++i) {
-        uint8_t &c = seen[host[i]];
-        shared += c == 1 ? 2u : c > 1 ? 1u : 0u;  // the first repeat counts both samples
-        c = uint8_t(c < 2 ? c + 1 : 2);
-    }
-    for (uint32_t i = 0; i < HYB_SAMPLES; ++i) seen[host[i]] = 0;
-    return shared * 5 <= HYB_SAMPLES;  // at most 20% of the samples in shared buckets
-}
 
 // qsort / keyvalue_qsort / argsort on the device: MSD-first for large arrays
 // of 4- and 8-byte keys, LSD otherwise (same arguments as run_sort)
@@ -725,14 +881,8 @@ int run_sort_auto(const void *keys_in, void *keys_out, const void *vals_in, void
             std::unique_lock<std::mutex> lock;
             int rc = acquire_workspace(s, HybPlan<KD, PL>::scratch_bytes(n), &ws, &lock);
             if (rc) return rc;
-            const bool nan_last = (KD::KIND == Kind::Float) && hasnan;
-            const uint64_t flip = desc ? ~0ull : 0ull;
-            if (hybrid_check() == 0 || hybrid_spread_ok<KD>(ws, keys_in, n, nan_last, flip, s))
-                return enqueue_hybrid<KD, PL>(ws, ws->data, const_cast<void *>(keys_in),
-                                              const_cast<void *>(vals_in), idx_out, n, hasnan, desc, canon, s);
-            // skewed data: the LSD path (its scratch fits in the workspace just grown)
-            return enqueue_sort<KD, PL>(ws, ws->data, keys_in, keys_out, vals_in, vals_out, idx_out, n, hasnan,
-                                        desc, canon, s);
+            return enqueue_hybrid<KD, PL>(ws, ws->data, const_cast<void *>(keys_in), const_cast<void *>(vals_in),
+                                          idx_out, n, hasnan, desc, canon, s);
         }
     }
     return run_sort<KD, PL>(keys_in, keys_out, vals_in, vals_out, idx_out, n, hasnan, desc, canon, s);

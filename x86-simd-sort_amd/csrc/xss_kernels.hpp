@@ -108,6 +108,20 @@ struct UpArgs {
     // (skip_ok: the pass may skip)
     uint64_t *andor;
     uint32_t skip_ok;
+    uint64_t *andor_slots;      // pass 0 (andor set): per-workgroup AND / OR, [gridDim.x][2]
+    // passes whose plan is made on the device (the MSD-first sort):
+    const uint32_t *shift_dev;  // the digit's shift (null: `shift`)
+    const uint32_t *gate;       // the pass runs only if *gate > gate_min (null: always)
+    uint32_t gate_min;
+    uint32_t nr_static;         // ranges walked by a bounded grid (0: one per workgroup)
+    uint32_t *wctr;             // ranges handed out by this counter (work stealing), or null
+    // per-segment AND / OR of the raw key bits (segmented passes over large
+    // buckets): accumulated here (seg_andor_out), or read to skip segments
+    // whose keys are all equal (seg_andor_in with seg_canon)
+    const uint32_t *rbseg;
+    uint64_t *seg_andor_out;
+    const uint64_t *seg_andor_in;
+    uint32_t seg_canon;
 };
 
 // Number of the earlier passes (of a sort that skips constant digits) that
@@ -139,6 +153,43 @@ __device__ __forceinline__ uint32_t moves_odd(const uint32_t *ran, uint32_t pass
     return __builtin_amdgcn_readfirstlane(moves_before(ran, pass) & 1u);
 }
 
+// A pass of a plan made on the device runs only if *gate > gate_min.  The
+// plan words are written by an earlier kernel on the stream, so plain
+// (cacheable, wave-uniform) loads see them.
+__device__ __forceinline__ bool gate_closed(const uint32_t *gate, uint32_t gate_min)
+{
+    return gate && __builtin_amdgcn_readfirstlane(*gate) <= gate_min;
+}
+__device__ __forceinline__ uint32_t dev_shift(const uint32_t *shift_dev, uint32_t shift)
+{
+    return shift_dev ? __builtin_amdgcn_readfirstlane(*shift_dev) : shift;
+}
+// A segment whose keys are all the same bits needs no pass at all: AND and
+// OR of its raw bits agree.  A NaN-canonicalising sort still rewrites a
+// segment of equal non-canonical NaNs.
+template <class KD>
+__device__ __forceinline__ bool segment_constant(const uint64_t *seg_andor, uint32_t seg, bool canon)
+{
+    if (!seg_andor) return false;
+    using U = typename KD::U;
+    // written by an earlier kernel (the first large-bucket histogram pass)
+    const U a = U(seg_andor[2 * size_t(seg)]), o = U(seg_andor[2 * size_t(seg) + 1]);
+    bool c = a == o;
+    if constexpr (KD::KIND == Kind::Float) {
+        if (c && canon && KD::is_nan(a) && a != KD::QNAN) c = false;
+    }
+    return __builtin_amdgcn_readfirstlane(c ? 1u : 0u) != 0u;
+}
+// The next range of a work-stealing walk (all threads call it): one device
+// atomic per range by thread 0, broadcast through LDS.
+__device__ __forceinline__ uint32_t next_range(uint32_t *wctr, uint32_t *slot)
+{
+    __syncthreads();  // the previous range's readers of *slot are done
+    if (threadIdx.x == 0) *slot = atomicAdd(wctr, 1u);
+    __syncthreads();
+    return *slot;
+}
+
 // Histogram copies of an upsweep workgroup (its waves share them round-robin;
 // more copies, fewer same-bank collisions of the LDS adds) and 16-byte loads
 // in flight per thread (build-time knobs).
@@ -148,7 +199,9 @@ __device__ __forceinline__ uint32_t moves_odd(const uint32_t *ran, uint32_t pass
 #ifndef XSS_UP_UNR
 #define XSS_UP_UNR 4
 #endif
-template <class KD>
+// SEG: the features of the large-bucket passes (per-segment AND / OR,
+// skipped segments, ranges from a counter), compiled out of the others.
+template <class KD, bool SEG = false>
 __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
 {
     using U = typename KD::U;
@@ -162,9 +215,17 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
     // compiled to take keys_alt unconditionally (hipcc, ROCm 7.2).
     __shared__ const void *s_kbuf[2];
     __shared__ uint64_t s_andor[2];
+    // a device-built table that came out empty: nothing to read (checked
+    // first, so that such a launch ends after one load)
+    if (a.nr_dev && *a.nr_dev == 0) return;
+    if (gate_closed(a.gate, a.gate_min)) return;
+    a.shift = dev_shift(a.shift_dev, a.shift);
     // a later pass over a digit every key shares reads nothing
     if (a.andor && a.pass > 0 && a.skip_ok && digit_constant(a.andor, int(a.shift))) return;
     const bool track = a.andor && a.pass == 0;  // pass 0: AND / OR of all order keys
+    const bool seg_track = SEG && a.seg_andor_out != nullptr;  // AND / OR of raw bits per segment
+    __shared__ uint64_t s_segao[2];
+    __shared__ uint32_t s_rg;
     if (threadIdx.x == 0) {
         s_kbuf[0] = a.keys;
         s_kbuf[1] = a.ran ? a.keys_alt : a.keys;
@@ -177,10 +238,20 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
     U t_and = U(~U(0)), t_or = U(0);
     // one range per workgroup, or (device-built tables, launched with a
     // bounded grid) every gridDim.x-th range
-    const uint32_t nr = a.nr_dev ? *a.nr_dev : gridDim.x;
-    for (uint32_t rg = blockIdx.x; rg < nr; rg += gridDim.x) {
+    const uint32_t nr = a.nr_dev ? *a.nr_dev : a.nr_static ? a.nr_static : gridDim.x;
+    if (nr == 0) return;  // an empty table: no counter traffic
+    const bool steal = SEG && a.wctr;
+    for (uint32_t rg = steal ? next_range(a.wctr, &s_rg) : blockIdx.x; rg < nr;
+         rg = steal ? next_range(a.wctr, &s_rg) : rg + gridDim.x) {
         __syncthreads();  // the previous range's histogram was written out
+        // a segment of equal keys takes no pass (its digit counts go unused)
+        if (SEG && a.seg_andor_in && segment_constant<KD>(a.seg_andor_in, a.rbseg[rg], a.seg_canon != 0)) continue;
         for (int i = threadIdx.x; i < NH * 256; i += UP_BLOCK) hh[i] = 0;
+        if (seg_track && threadIdx.x == 0) {
+            s_segao[0] = ~0ull;
+            s_segao[1] = 0ull;
+        }
+        U r_and = U(~U(0)), r_or = U(0);
         __syncthreads();
         const U flip = U(a.flip);
         const bool nl = a.nan_last != 0;
@@ -204,6 +275,10 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
                     const U o = KD::ord(e[j], flip, nl);
                     t_and &= o;
                     t_or |= o;
+                }
+                if (seg_track) {
+                    r_and &= e[j];
+                    r_or |= e[j];
                 }
                 const uint32_t d = KD::digit(e[j], flip, nl, shift);
                 const uint64_t act = __ballot(1);
@@ -235,7 +310,16 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
                 t_and &= o;
                 t_or |= o;
             }
+            if (seg_track) {
+                r_and &= keys[i];
+                r_or |= keys[i];
+            }
             atomicAdd(&h[KD::digit(keys[i], flip, nl, shift)], 1u);
+        }
+        if (seg_track) {
+            atomicAnd(reinterpret_cast<unsigned long long *>(&s_segao[0]),
+                      (unsigned long long)r_and | ~(unsigned long long)U(~U(0)));
+            atomicOr(reinterpret_cast<unsigned long long *>(&s_segao[1]), (unsigned long long)r_or);
         }
         __syncthreads();
         for (int i = threadIdx.x; i < 256; i += UP_BLOCK) {
@@ -244,15 +328,46 @@ __global__ __launch_bounds__(UP_BLOCK) void upsweep_kernel(UpArgs a)
             for (int j = 0; j < NH; ++j) c += hh[j * 256 + i];
             a.rsum[size_t(i) * a.ranges + rg] = c;
         }
+        if (seg_track && threadIdx.x == 0) {
+            uint64_t *so = a.seg_andor_out + 2 * size_t(a.rbseg[rg]);
+            atomicAnd(reinterpret_cast<unsigned long long *>(so), (unsigned long long)s_segao[0]);
+            atomicOr(reinterpret_cast<unsigned long long *>(so + 1), (unsigned long long)s_segao[1]);
+        }
     }
     if (track) {
         atomicAnd(reinterpret_cast<unsigned long long *>(&s_andor[0]), (unsigned long long)t_and | ~(unsigned long long)U(~U(0)));
         atomicOr(reinterpret_cast<unsigned long long *>(&s_andor[1]), (unsigned long long)t_or);
         __syncthreads();
+        // one slot per workgroup (a later kernel folds them): thousands of
+        // device atomics on one address would serialise
         if (threadIdx.x == 0) {
-            atomicAnd(reinterpret_cast<unsigned long long *>(a.andor), (unsigned long long)s_andor[0]);
-            atomicOr(reinterpret_cast<unsigned long long *>(a.andor + 1), (unsigned long long)s_andor[1]);
+            a.andor_slots[2 * blockIdx.x] = s_andor[0];
+            a.andor_slots[2 * blockIdx.x + 1] = s_andor[1];
         }
+    }
+}
+
+// AND / OR of the order keys from the upsweep's per-workgroup slots (all
+// threads of the calling workgroup; the result in out[0], out[1])
+__device__ __forceinline__ void fold_andor(const uint64_t *slots, uint32_t nslots, uint64_t *out)
+{
+    __shared__ unsigned long long s_ao[2];
+    if (threadIdx.x == 0) {
+        s_ao[0] = ~0ull;
+        s_ao[1] = 0ull;
+    }
+    __syncthreads();
+    unsigned long long x = ~0ull, y = 0ull;
+    for (uint32_t i = threadIdx.x; i < nslots; i += blockDim.x) {
+        x &= slots[2 * i];
+        y |= slots[2 * i + 1];
+    }
+    atomicAnd(&s_ao[0], x);
+    atomicOr(&s_ao[1], y);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = s_ao[0];
+        out[1] = s_ao[1];
     }
 }
 
@@ -271,6 +386,8 @@ struct ScanArgs {
     uint32_t *rpre;        // [256][ranges] exclusive prefix over ranges, per digit
     uint32_t *total;       // [256] keys per digit
     uint32_t ranges;
+    const uint64_t *andor_slots;  // pass 0 of a sort that skips constant digits: fold the
+    uint64_t *andor;              // upsweep's per-workgroup AND / OR into andor (workgroup 0)
 };
 
 // workgroup d scans digit d's column (ranges <= 16 * SCAN_BLOCK)
@@ -307,6 +424,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void range_scan_kernel(ScanArgs a)
         run += v[i];
     }
     if (t == SCAN_BLOCK - 1) a.total[d] = run;
+    if (a.andor_slots && d == 0) fold_andor(a.andor_slots, a.ranges, a.andor);
 }
 
 // exclusive scan of the 256 digit totals by the first 256 threads of a
@@ -364,6 +482,14 @@ struct PassArgs {
     uint32_t pass;
     uint32_t skip_ok;
     const uint64_t *andor;  // AND / OR of the order keys (pass 0's upsweep), or null
+    const uint32_t *shift_dev;  // plans made on the device: the digit's shift (null: `shift`)
+    const uint32_t *gate;       // and the pass runs only if *gate > gate_min (null: always)
+    uint32_t gate_min;
+    // a pass that is the last only if the plan says so: last = (*last_gate
+    // == last_q + 1), and it then writes canonical NaNs if canon_if_last
+    const uint32_t *last_gate;
+    uint32_t last_q;
+    uint32_t canon_if_last;
 };
 
 template <class KD, int PL, int BLOCK, int ITEMS>
@@ -607,7 +733,9 @@ __global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL, BLOCK) void do
     const uint32_t ntiles = (a.n + TILE - 1) / TILE;
     const uint32_t tile0 = blockIdx.x * a.range_tiles;
     const uint32_t tile1 = min(ntiles, tile0 + a.range_tiles);
+    if (gate_closed(a.gate, a.gate_min)) return;
     PassArgs b = a;
+    b.shift = dev_shift(a.shift_dev, a.shift);
     if (a.ran) {
         // all n keys in one digit: a stable pass leaves them in place
         const bool constant = a.andor ? digit_constant(a.andor, int(a.shift))
@@ -680,6 +808,9 @@ struct SegScanArgs {
     uint32_t nr;             // ranges (when nr_dev is null)
     uint32_t stride;         // row length of rsum / rpre
     const uint32_t *nr_dev;  // device-built tables: the range count
+    const uint32_t *gate;    // the pass runs only if *gate > gate_min (null: always)
+    uint32_t gate_min;
+    uint32_t *wctr_reset;    // work-stealing counters of the pass's downsweep, cleared here
 };
 
 // workgroup d: segmented exclusive scan of column d over the ranges
@@ -688,6 +819,8 @@ __global__ __launch_bounds__(256) void seg_scan_kernel(SegScanArgs a)
 {
     __shared__ uint32_t s_sum[256];
     __shared__ uint32_t s_flag[256];
+    if (gate_closed(a.gate, a.gate_min)) return;
+    if (a.wctr_reset && blockIdx.x == 0 && threadIdx.x == 0) *a.wctr_reset = 0;
     const uint32_t d = blockIdx.x, t = threadIdx.x;
     const uint32_t nr = a.nr_dev ? *a.nr_dev : a.nr;
     const uint32_t per = (nr + 255) / 256;
@@ -741,6 +874,13 @@ struct SegRanges {
     uint32_t nr;
     uint32_t stride;
     const uint32_t *nr_dev;  // device-built tables: the range count
+    uint32_t *wctr;          // ranges handed out by this counter (work stealing), or null
+    // skip segments whose keys are all equal (segment_constant); the last
+    // pass of an argsort then writes such a segment's indices from home_vals,
+    // where every pass left them
+    const uint64_t *seg_andor;
+    uint32_t seg_canon;
+    const void *home_vals;
 };
 
 // Tile shape of the segmented passes: 512 threads x 16 items (8 for items
@@ -763,19 +903,47 @@ struct SegShape {
     static constexpr int WAVES = BYTES == 4 ? 4 : 1;
 };
 
-template <class KD, int PL, int BLOCK, int ITEMS, bool NL, bool UNST = false>
+// SEG: the features of the large-bucket passes (plan gates, the last pass
+// chosen on the device, skipped segments, ranges from a counter).
+template <class KD, int PL, int BLOCK, int ITEMS, bool NL, bool UNST = false, bool SEG = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(SegShape<KD, PL>::WAVES)))
 void seg_downsweep_kernel(PassArgs a, SegRanges r)
 {
     constexpr int TILE = BLOCK * ITEMS;
     __shared__ DownsweepSmem<KD, PL, BLOCK, ITEMS> sm;
+    __shared__ uint32_t s_b;
     const uint32_t tid = threadIdx.x;
-    // one range per workgroup, or (device-built tables, launched with a
-    // bounded grid) every gridDim.x-th range
     const uint32_t nr = r.nr_dev ? *r.nr_dev : r.nr;
-    for (uint32_t b = blockIdx.x; b < nr; b += gridDim.x) {
+    if (nr == 0) return;  // an empty table
+    if constexpr (SEG) {
+        if (gate_closed(a.gate, a.gate_min)) return;
+        if (a.last_gate) {
+            a.last = dev_shift(a.last_gate, 0) == a.last_q + 1;
+            a.canon_nan = a.last ? a.canon_if_last : 0u;
+        }
+    }
+    a.shift = dev_shift(a.shift_dev, a.shift);
+    // one range per workgroup, or (device-built tables, launched with a
+    // bounded grid) every gridDim.x-th range, or ranges taken from a counter
+    const bool steal = SEG && r.wctr;
+    for (uint32_t b = steal ? next_range(r.wctr, &s_b) : blockIdx.x; b < nr;
+         b = steal ? next_range(r.wctr, &s_b) : b + gridDim.x) {
         __syncthreads();  // the previous range's LDS reads are done
         const uint64_t rb = r.begin[b], re = r.end[b], sbase = r.sbase[b];
+        if (SEG && r.seg_andor && segment_constant<KD>(r.seg_andor, r.rbseg[b], r.seg_canon != 0)) {
+            // all keys equal: the segment stays where it is; an argsort's
+            // last pass writes its indices (in order: every pass is stable)
+            if constexpr (PL == PL_IDX) {
+                if (a.last) {
+                    const uint32_t *hv = static_cast<const uint32_t *>(r.home_vals);
+                    for (uint64_t i = rb + tid; i < re; i += BLOCK) {
+                        if (a.idx32_out) a.idx32_out[i] = hv[i] - uint32_t(sbase);
+                        else a.idx_out[i] = uint64_t(hv[i]);
+                    }
+                }
+            }
+            continue;
+        }
         // digit starts inside this range's segment
         const uint32_t dstart = digit_starts(r.segtot + size_t(r.rbseg[b]) * 256, sm.wsum);
         uint64_t run = tid < 256 ? sbase + dstart + r.rpre[size_t(tid) * r.stride + b] : 0u;

@@ -68,13 +68,13 @@ int acquire_workspace(hipStream_t stream, size_t data_bytes, Workspace **out, st
                              data_bytes);
         if (need_meta) XSS_HIP_CHECK(hipMalloc(&ws->meta, META_BYTES));
         if (grow_data) {
-            // earlier work on this stream may still use the old buffer
-            XSS_HIP_CHECK(hipStreamSynchronize(stream));
-            if (ws->data) XSS_HIP_CHECK(hipFree(ws->data));
+            // stream-ordered: the old buffer is released once earlier work on
+            // this stream is done, with no host wait
+            if (ws->data) XSS_HIP_CHECK(hipFreeAsync(ws->data, stream));
             ws->data = nullptr;
             ws->data_cap = 0;
             size_t cap = round_up(data_bytes + data_bytes / 4, 1 << 20);
-            XSS_HIP_CHECK(hipMalloc(&ws->data, cap));
+            XSS_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&ws->data), cap, stream));
             ws->data_cap = cap;
         }
     }
@@ -107,12 +107,11 @@ int acquire_aux(hipStream_t stream, size_t bytes, char **out, std::unique_lock<s
                              "staging buffer must grow to %zu bytes during stream capture; "
                              "run the call once before capturing",
                              bytes);
-        XSS_HIP_CHECK(hipStreamSynchronize(stream));
-        if (ws->aux) XSS_HIP_CHECK(hipFree(ws->aux));
+        if (ws->aux) XSS_HIP_CHECK(hipFreeAsync(ws->aux, stream));
         ws->aux = nullptr;
         ws->aux_cap = 0;
         const size_t cap = round_up(bytes + bytes / 4, 1 << 20);
-        XSS_HIP_CHECK(hipMalloc(&ws->aux, cap));
+        XSS_HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&ws->aux), cap, stream));
         ws->aux_cap = cap;
     }
     *out = ws->aux;
@@ -216,10 +215,10 @@ int xss_gpu_release_workspaces(void)
         int cur = 0;
         XSS_HIP_CHECK(hipGetDevice(&cur));
         XSS_HIP_CHECK(hipSetDevice(ws->device));
+        if (ws->data) XSS_HIP_CHECK(hipFreeAsync(ws->data, ws->stream));
+        if (ws->aux) XSS_HIP_CHECK(hipFreeAsync(ws->aux, ws->stream));
         XSS_HIP_CHECK(hipStreamSynchronize(ws->stream));
         if (ws->meta) XSS_HIP_CHECK(hipFree(ws->meta));
-        if (ws->data) XSS_HIP_CHECK(hipFree(ws->data));
-        if (ws->aux) XSS_HIP_CHECK(hipFree(ws->aux));
         ws->meta = ws->data = ws->aux = nullptr;
         ws->data_cap = ws->aux_cap = 0;
         XSS_HIP_CHECK(hipSetDevice(cur));

@@ -19,7 +19,8 @@ namespace xss {
 
 constexpr size_t CTRL_BYTES = 256;
 constexpr size_t RSUM_BYTES = size_t(MAX_RANGES) * 256 * sizeof(uint32_t);
-constexpr size_t META_BYTES = CTRL_BYTES + 2 * RSUM_BYTES + 1024 + 256;
+constexpr size_t ANDOR_SLOTS_BYTES = size_t(MAX_RANGES) * 16;
+constexpr size_t META_BYTES = CTRL_BYTES + 2 * RSUM_BYTES + 1024 + 256 + ANDOR_SLOTS_BYTES;
 
 struct Workspace {
     int device = 0;
@@ -39,6 +40,11 @@ struct Workspace {
     uint32_t *ran() const { return reinterpret_cast<uint32_t *>(meta + CTRL_BYTES + 2 * RSUM_BYTES + 1024); }
     // AND / OR of the order keys of such a sort (after the 8 pass flags)
     uint64_t *andor() const { return reinterpret_cast<uint64_t *>(meta + CTRL_BYTES + 2 * RSUM_BYTES + 1024 + 64); }
+    // per-workgroup AND / OR of pass 0's upsweep, folded into andor() by the scan
+    uint64_t *andor_slots() const
+    {
+        return reinterpret_cast<uint64_t *>(meta + CTRL_BYTES + 2 * RSUM_BYTES + 1024 + 256);
+    }
 };
 
 // Returns the workspace of (current device, stream), locked, with at least

@@ -144,6 +144,7 @@ int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *
     u.keys = src_k; u.n = nn; u.range_keys = uint32_t(rtiles * Plan::tile(n)); u.shift = uint32_t(shift);
     u.nan_last = nan_last; u.flip = flip; u.rsum = ws->rsum(); u.ranges = ranges;
     u.keys_alt = dst_k; u.ran = ran; u.pass = pass; u.andor = andor; u.skip_ok = skip_ok;
+    u.andor_slots = ws->andor_slots();
     {
         KernelTimer t("upsweep", s);
         hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
@@ -151,6 +152,10 @@ int enqueue_lsd_pass(Workspace *ws, const void *src_k, const void *src_v, void *
     XSS_HIP_CHECK(hipGetLastError());
     ScanArgs sc{};
     sc.rsum = ws->rsum(); sc.rpre = ws->rpre(); sc.total = ws->total(); sc.ranges = ranges;
+    if (andor && pass == 0) {
+        sc.andor_slots = ws->andor_slots();
+        sc.andor = andor;
+    }
     {
         KernelTimer t("range_scan", s);
         hipLaunchKernelGGL(range_scan_kernel<0>, dim3(256), dim3(SCAN_BLOCK), 0, s, sc);

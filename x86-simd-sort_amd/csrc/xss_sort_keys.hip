@@ -36,11 +36,6 @@ std::atomic<uint64_t> &hybrid_min_n()
     static std::atomic<uint64_t> v{HYB_MIN_N};
     return v;
 }
-std::atomic<uint64_t> &hybrid_check()
-{
-    static std::atomic<uint64_t> v{1};
-    return v;
-}
 std::atomic<uint64_t> &count_min_n()
 {
     static std::atomic<uint64_t> v{CS_MIN_N};
@@ -53,10 +48,6 @@ extern "C" int xss_gpu_set_tuning(const char *name, uint64_t value)
     if (!name) return set_error(XSS_ERR_INVALID, "null tuning name");
     if (std::string(name) == "hybrid_min_n") {
         hybrid_min_n().store(value);
-        return XSS_OK;
-    }
-    if (std::string(name) == "hybrid_check") {
-        hybrid_check().store(value);
         return XSS_OK;
     }
     if (std::string(name) == "count_min_n") {

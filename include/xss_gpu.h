@@ -24,7 +24,9 @@
  * argsort/argselect write n size_t (uint64_t) indices to `arg` and leave the
  * keys untouched, like the reference (src/x86simdsort-static-incl.h:92-121).
  *
- * Limits: n < 2^30 per array.  Host-pointer forms are the C++ templates of
+ * Limits: n <= 4,278,190,080 (2^32 - 2^24) keys per array (positions are
+ * 32-bit on the device); a larger n returns XSS_ERR_INVALID.  The reference
+ * takes any size_t.  Host-pointer forms are the C++ templates of
  * x86simdsort.h and the 8 keyvalue_qsort_* symbols declared at the end.
  */
 #ifndef XSS_GPU_H
@@ -73,7 +75,10 @@ XSS_API int xss_gpu_timing_enable(int on);
  *                   most significant varying digits in LDS); default 2^25.
  *   "count_min_n"   qsort of 16-bit keys counts keys per value (a counting
  *                   sort: one read, one write) for arrays of at least this
- *                   many keys; default 2^23. */
+ *                   many keys; default 2^23.
+ *   "workspace_limit" bytes a stream's workspace may grow to (0, the
+ *                   default: no limit); a call that would need more fails
+ *                   with XSS_ERR_NOMEM, like a refused allocation. */
 XSS_API int xss_gpu_set_tuning(const char *name, uint64_t value);
 XSS_API int xss_gpu_timing_collect(int max_names, const char **names, double *total_ms, uint64_t *counts);
 

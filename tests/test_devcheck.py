@@ -153,3 +153,26 @@ def test_kv_partial_checker():
     bv[j], bv[far] = gv[far], gv[j]
     with pytest.raises(AssertionError):
         T.assert_kv_partial(bk, bv, 2500, k, v, ek, ev, "u32", "cut run")
+
+
+def test_check_sorted_perm_in_chunks(monkeypatch):
+    """The chunked 16-bit path (arrays beyond one torch.sort) accepts a sort
+    and rejects a swap and a changed key; run here with small chunks."""
+    monkeypatch.setattr(D, "CHUNK", 1000)
+    monkeypatch.setattr(D, "SORT_MAX", 999)
+    O = T.ORACLE()
+    a = T.gen("i16", "dups", 5555, 3)
+    good = t_of(O.qsort(a, "i16"))
+    D.check_sorted_perm(good, t_of(a), "i16", False, "good")
+    o = D.order_key(good, "i16")
+    i = int(torch.nonzero(o[1:] != o[:-1])[2])
+    bad = good.clone()
+    bad[i], bad[i + 1] = good[i + 1], good[i]
+    with pytest.raises(AssertionError):
+        D.check_sorted_perm(bad, t_of(a), "i16", False, "swap")
+    bad = good.clone()
+    bad[-1] = bad[0]
+    with pytest.raises(AssertionError):
+        D.check_sorted_perm(bad, t_of(a), "i16", False, "changed")
+    idx = torch.randperm(5555)
+    assert torch.equal(D.gather(good, idx), good[idx])

@@ -217,3 +217,41 @@ def test_plumbing_config_1m(xss):
     R = T.REF()
     if R is not None:
         T.assert_sorted_equal(got, R.qsort(a, "f32"), "f32", "1M vs reference", zeros_as_values=True)
+
+
+BEYOND = [("qsort", "f32", "random", (1 << 30) + 4097), ("argsort", "u32", "dups", (1 << 30) + 17),
+          ("kv", "f32", "random", (1 << 30) + 5), ("qsort", "u16", "random", (1 << 31) + 3),
+          ("argsort", "i16", "random", (1 << 31) + 3), ("qselect", "f32", "nan", (1 << 30) + 9),
+          ("qsort", "u64", "random", (1 << 29) + 1)]
+
+
+@pytest.mark.parametrize("op,tag,recipe,n", BEYOND)
+def test_beyond_2_30_keys(xss, op, tag, recipe, n):
+    """Arrays above 2^30 keys (up to 2^31 16-bit keys): the per-call limit is
+    2^32 - 2^24 keys, as 32-bit device positions allow."""
+    inp = D.gen(tag, recipe, n, 777)
+    hasnan = recipe == "nan"
+    what = f"{op}<{tag}> {recipe} n={n}"
+    print(what, flush=True)
+    if op == "qsort":
+        got = inp.clone()
+        xss.qsort(darray(xss, got, tag), hasnan, False)
+        D.check_sort(got, inp, tag, hasnan, False, True, what)
+    elif op == "argsort":
+        idx = xss.argsort(darray(xss, inp, tag), hasnan, True)
+        i64 = idx.t[: 8 * n].view(torch.int64)
+        D.check_permutation(i64, n, what)
+        D.check_sort(D.gather(inp, i64), inp, tag, hasnan, True, False, what)
+    elif op == "kv":
+        gk = inp.clone()
+        gv = torch.arange(n, device="cuda", dtype=torch.int32)
+        xss.keyvalue_qsort(darray(xss, gk, tag), darray(xss, gv, "u32"), hasnan, False)
+        D.check_gather(gk, inp, gv.to(torch.int64), what)
+        D.check_sort(gk, inp, tag, hasnan, False, False, what)
+    else:
+        got = inp.clone()
+        k = n // 2
+        xss.qselect(darray(xss, got, tag), k, hasnan)
+        og, srt = D.order_key(got, tag), torch.sort(D.order_key(inp, tag)).values
+        assert og[k] == srt[k], f"{what}: k-th value"
+        assert bool((og[:k] <= srt[k]).all()) and bool((og[k + 1:] >= srt[k]).all()), f"{what}: partition"

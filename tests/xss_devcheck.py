@@ -104,9 +104,39 @@ def gen(tag, recipe, n, seed, device="cuda"):
     raise ValueError(recipe)
 
 
+SORT_MAX = (1 << 31) - 1  # torch sorts and indexes at most INT_MAX elements at once
+CHUNK = 1 << 30
+
+
+def _counts16(o):
+    """Histogram of 16-bit order keys (65536 bins), chunk by chunk."""
+    c = torch.zeros(1 << 16, dtype=torch.int64, device=o.device)
+    for i in range(0, o.numel(), CHUNK):
+        c += torch.bincount(o[i:i + CHUNK].to(torch.int32) + (1 << 15), minlength=1 << 16)
+    return c
+
+
+def gather(x, idx):
+    """x[idx] for index tensors of any length."""
+    if idx.numel() <= SORT_MAX:
+        return x[idx]
+    return torch.cat([x[idx[i:i + CHUNK]] for i in range(0, idx.numel(), CHUNK)])
+
+
 def check_sorted_perm(got, inp, tag, desc, what):
     """got is inp ordered by the reference's comparator (value classes)."""
     og = order_key(got, tag, desc)
+    if og.numel() > SORT_MAX:
+        # beyond one torch.sort: 16-bit keys, checked as ordered plus equal
+        # counts of every order key
+        assert WIDTH[tag] == 16, "arrays over 2^30 keys are checked for 16-bit keys only"
+        for i in range(0, og.numel() - 1, CHUNK):
+            seg = og[i:i + CHUNK + 1]
+            if not bool((seg[1:] >= seg[:-1]).all()):
+                raise AssertionError(f"{what}: not ordered near {i}")
+        if not torch.equal(_counts16(og), _counts16(order_key(inp, tag, desc))):
+            raise AssertionError(f"{what}: not a permutation of the input")
+        return
     oi = torch.sort(order_key(inp, tag, desc)).values
     if not torch.equal(og, oi):
         bad = torch.nonzero(og != oi)[0].item()

@@ -4,6 +4,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <atomic>
 #include <map>
 #include <memory>
 #include <string>
@@ -33,6 +34,12 @@ int hip_fail(hipError_t e, const char *what)
                      hipGetErrorString(e));
 }
 
+std::atomic<uint64_t> &workspace_limit()
+{
+    static std::atomic<uint64_t> v{0};
+    return v;
+}
+
 namespace {
 std::mutex g_map_mu;
 std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
@@ -59,6 +66,10 @@ int acquire_workspace(hipStream_t stream, size_t data_bytes, Workspace **out, st
     const bool need_meta = ws->meta == nullptr;
     const bool grow_data = data_bytes > ws->data_cap;
     if (need_meta || grow_data) {
+        const uint64_t lim = workspace_limit().load();
+        if (lim && META_BYTES + data_bytes > lim)
+            return set_error(XSS_ERR_NOMEM, "workspace of %zu bytes exceeds the limit of %llu bytes",
+                             META_BYTES + data_bytes, (unsigned long long)lim);
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         XSS_HIP_CHECK(hipStreamIsCapturing(stream, &cs));
         if (cs != hipStreamCaptureStatusNone)

@@ -11,6 +11,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <mutex>
 
 #include "xss_kernels.hpp"
@@ -57,6 +58,11 @@ int acquire_workspace(hipStream_t stream, size_t data_bytes, Workspace **out, st
 int acquire_aux(hipStream_t stream, size_t bytes, char **out, std::unique_lock<std::mutex> *lock);
 
 int set_error(int code, const char *fmt, ...);
+
+// Largest workspace a stream may grow to (xss_gpu_set_tuning
+// "workspace_limit", 0 = no limit): growth beyond it fails with
+// XSS_ERR_NOMEM, as a refused allocation does.
+std::atomic<uint64_t> &workspace_limit();
 
 // Optional per-kernel timing (xss_gpu_timing_enable): HIP events recorded
 // on the launch stream around each kernel, summed per kernel name.

@@ -17,7 +17,10 @@
 
 namespace xss {
 
-constexpr uint32_t MAX_N = (1u << 30) - 1;
+// Largest array of one call: positions, ranks and digit counts are 32-bit
+// on the device, and tile / range arithmetic needs 2^24 of headroom below
+// 2^32 (4,278,190,080 keys: 16 GiB of 4-byte keys, 32 GiB of 8-byte ones).
+constexpr uint32_t MAX_N = 0xFF000000u;
 
 template <class T>
 struct TypeKey;
@@ -269,7 +272,8 @@ int run_sort(const void *keys_in, void *keys_out, const void *vals_in, void *val
              size_t n, bool hasnan, bool desc, bool canon, hipStream_t s)
 {
     if (n > MAX_N) return set_error(XSS_ERR_INVALID, "n = %zu exceeds the per-call limit %u", n, MAX_N);
-    if (n == 0) return XSS_OK;
+    // n <= 1 is a no-op (src/xss-common-qsort.h:660); argsort of one key still writes index 0
+    if (n == 0 || (n == 1 && PL != PL_IDX)) return XSS_OK;
     if (!keys_in || (PL == PL_IDX ? !idx_out : !keys_out) || ((PL == PL_V32 || PL == PL_V64) && (!vals_in || !vals_out)))
         return set_error(XSS_ERR_INVALID, "null pointer argument");
     using Plan = SortPlan<KD, PL>;

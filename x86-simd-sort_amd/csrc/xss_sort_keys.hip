@@ -50,6 +50,10 @@ extern "C" int xss_gpu_set_tuning(const char *name, uint64_t value)
         hybrid_min_n().store(value);
         return XSS_OK;
     }
+    if (std::string(name) == "workspace_limit") {
+        workspace_limit().store(value);
+        return XSS_OK;
+    }
     if (std::string(name) == "count_min_n") {
         count_min_n().store(value);
         return XSS_OK;

@@ -136,6 +136,154 @@ __global__ __launch_bounds__(256) void hyb_finish_kernel(const uint32_t *plan, v
     }
 }
 
+// ---------------------------------------------------------------- two-digit histogram of pass A
+//
+// Pass A's histogram read counts the keys of its range by both MSD digits
+// at once (65536 bins of 16 bits in LDS), so that pass B, whose segments are
+// the top buckets, needs no histogram read of its own: the keys of pass-A
+// range r with top digit d1 land in one contiguous run of Y, and a pass-B
+// range made of consecutive such runs has, as its digit counts, the sum of
+// their rows.  Pass A uses H2_RANGES ranges (one workgroup per CU).
+#ifndef XSS_H2_RANGES
+#define XSS_H2_RANGES 256  // pass-A ranges (build-time knob)
+#endif
+constexpr uint32_t H2_RANGES = XSS_H2_RANGES;
+constexpr uint32_t H2_BLOCK = 1024;
+constexpr uint32_t H2_SPILL = 0x4000;  // a 16-bit bin reaching this moves it to the spill log
+
+struct H2Args {
+    const void *keys;
+    uint32_t n;
+    uint32_t range_keys;
+    uint32_t ranges;
+    uint32_t shift_hi;          // shift of the high digit (null shift_dev: the top digit)
+    const uint32_t *shift_dev;  // the plan's pass-A digit
+    const uint32_t *gate;       // runs only if *gate > 0 (null: always)
+    uint64_t flip;
+    uint32_t nan_last;
+    uint64_t *andor_slots;      // first launch: per-workgroup AND / OR of the order keys
+    uint32_t *res;              // [ranges][32768]: two 16-bit bins per word, bin = (hi << 8) | lo
+    uint32_t *spill_cnt;        // [ranges]
+    uint32_t *spill_bin;        // [ranges][spill_cap]: bins that moved H2_SPILL keys out of LDS
+    uint32_t spill_cap;
+    uint32_t *rsum;             // [256][ranges] high-digit counts (pass A's range scan)
+};
+
+template <class KD>
+__global__ __launch_bounds__(H2_BLOCK) void hyb_h2_kernel(H2Args a)
+{
+    using U = typename KD::U;
+    __shared__ uint32_t h[32768];
+    __shared__ unsigned long long s_ao[2];
+    if (a.gate && __builtin_amdgcn_readfirstlane(*a.gate) == 0) return;
+    const uint32_t shift_hi = dev_shift(a.shift_dev, a.shift_hi);
+    const int shift = int(shift_hi) - 8;  // the 16-bit bin: high digit and the one below it
+    const uint32_t r = blockIdx.x, tid = threadIdx.x;
+    for (uint32_t i = tid; i < 32768; i += H2_BLOCK) h[i] = 0;
+    if (tid == 0) {
+        __hip_atomic_store(a.spill_cnt + r, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_ao[0] = ~0ull;
+        s_ao[1] = 0ull;
+    }
+    __syncthreads();
+    const bool track = a.andor_slots != nullptr;
+    const U flip = U(a.flip);
+    const bool nl = a.nan_last != 0;
+    const U *keys = static_cast<const U *>(a.keys);
+    const uint32_t r0 = min(a.n, r * a.range_keys), r1 = uint32_t(min<uint64_t>(a.n, uint64_t(r0) + a.range_keys));
+    U t_and = U(~U(0)), t_or = U(0);
+    // an add of c to bin b found `word`: crossing H2_SPILL moves H2_SPILL
+    // keys out of the bin into the spill log.  One move per bin is pending
+    // at a time and other waves add at most a few thousand meanwhile, so a
+    // bin never carries into its neighbour.
+    auto add = [&](uint32_t b, uint32_t c) {
+        const uint32_t sh = (b & 1u) * 16u;
+        const uint32_t w = atomicAdd(&h[b >> 1], c << sh);
+        const uint32_t old = (w >> sh) & 0xffffu;
+        if (old < H2_SPILL && old + c >= H2_SPILL) {
+            atomicSub(&h[b >> 1], H2_SPILL << sh);
+            const uint32_t i = atomicAdd(&a.spill_cnt[r], 1u);
+            if (i < a.spill_cap)
+                __hip_atomic_store(a.spill_bin + size_t(r) * a.spill_cap + i, b, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+    };
+    auto count = [&](U x) {
+        const U o = KD::ord(x, flip, nl);
+        if (track) {
+            t_and &= o;
+            t_or |= o;
+        }
+        const uint32_t b = uint32_t(o >> shift) & 0xffffu;
+        const uint64_t act = __ballot(1);
+        const uint32_t b0 = __builtin_amdgcn_readfirstlane(b);
+        if (__ballot(b != b0) == 0) {  // a wave of one bin (sorted or constant runs): one add
+            if (lanes_below(act) == 0) add(b0, uint32_t(__popcll(act)));
+        } else {
+            add(b, 1u);
+        }
+    };
+    constexpr int VEC = 16 / sizeof(U);
+    const uint32_t mis = uint32_t(reinterpret_cast<uintptr_t>(keys + r0) & 15u);
+    const uint32_t head = min(r1 - r0, uint32_t(((16u - mis) & 15u) / sizeof(U)));
+    const uint32_t v0 = r0 + head;
+    const uint32_t nvec = (r1 - v0) / VEC;
+    const uint4 *kv = reinterpret_cast<const uint4 *>(keys + v0);
+    constexpr uint32_t UNR = 4;
+    uint32_t v = tid;
+    for (; v + (UNR - 1) * H2_BLOCK < nvec; v += UNR * H2_BLOCK) {
+        uint4 q[UNR];
+#pragma unroll
+        for (uint32_t u = 0; u < UNR; ++u) q[u] = kv[v + u * H2_BLOCK];
+#pragma unroll
+        for (uint32_t u = 0; u < UNR; ++u) {
+            const U *e = reinterpret_cast<const U *>(&q[u]);
+#pragma unroll
+            for (int j = 0; j < VEC; ++j) count(e[j]);
+        }
+    }
+    for (; v < nvec; v += H2_BLOCK) {
+        const uint4 q = kv[v];
+        const U *e = reinterpret_cast<const U *>(&q);
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) count(e[j]);
+    }
+    const uint32_t t0 = v0 + nvec * VEC;
+    const uint32_t rest = head + (r1 - t0);
+    for (uint32_t i = tid; i < rest; i += H2_BLOCK) count(keys[i < head ? r0 + i : t0 + (i - head)]);
+    if (track) {
+        atomicAnd(&s_ao[0], (unsigned long long)t_and | ~(unsigned long long)U(~U(0)));
+        atomicOr(&s_ao[1], (unsigned long long)t_or);
+    }
+    __syncthreads();
+    if (track && tid == 0) {
+        a.andor_slots[2 * r] = s_ao[0];
+        a.andor_slots[2 * r + 1] = s_ao[1];
+    }
+    uint32_t *out = a.res + size_t(r) * 32768;
+    for (uint32_t i = tid; i < 32768; i += H2_BLOCK) out[i] = h[i];
+    // high-digit counts: four threads per digit (64 bins each), plus spills
+    const uint32_t d1 = tid >> 2, qtr = tid & 3u;
+    uint32_t c = 0;
+    for (uint32_t i = 0; i < 32; ++i) {
+        const uint32_t w = h[d1 * 128 + qtr * 32 + i];
+        c += (w & 0xffffu) + (w >> 16);
+    }
+    c += __shfl_xor(c, 1);
+    c += __shfl_xor(c, 2);
+    if (qtr == 0) {
+        // the log was written by other threads' device atomics and stores
+        // in this kernel: read past the CU's cache
+        const uint32_t ns = min(__hip_atomic_load(a.spill_cnt + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                                a.spill_cap);
+        for (uint32_t i = 0; i < ns; ++i)
+            if ((__hip_atomic_load(a.spill_bin + size_t(r) * a.spill_cap + i, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT) >> 8) == d1)
+                c += H2_SPILL;
+        a.rsum[size_t(d1) * a.ranges + r] = c;
+    }
+}
+
 // ranges of pass B from pass A's digit totals (one workgroup)
 template <int = 0>
 __global__ __launch_bounds__(256) void hyb_top_ranges_kernel(HybArgs a)
@@ -194,6 +342,82 @@ __global__ __launch_bounds__(256) void hyb_top_ranges_kernel(HybArgs a)
         a.tb.sbase[r] = st;
         a.tb.bseg[r] = lo;
         a.tb.head[r] = b == st ? 1u : 0u;
+    }
+}
+
+// Pass B's per-range digit counts without a histogram read of Y.  The keys
+// of pass-A range r with top digit d1 lie in one run of Y inside top bucket
+// d1, at offset rpre_a[d1][r] (pass A's scan); the two-digit histogram of
+// pass A already counts each run's keys by the next digit.  A pass-B range
+// [a, b) of bucket d1 takes the rows of the runs it covers whole and counts
+// the keys of the (at most two) runs it covers in part directly from Y --
+// about a run's length per range.  A workgroup per range.
+constexpr uint32_t GRP_MAX_RUNS = 4096;
+template <class KD>
+__global__ __launch_bounds__(1024) void hyb_msd_hist_kernel(HybTable tb, const void *ykeys_v, uint64_t flip64,
+                                                            uint32_t nan_last, const uint32_t *shift_dev,
+                                                            const uint32_t *rpre_a, const uint32_t *top_total,
+                                                            uint32_t ranges_a, const uint32_t *res,
+                                                            const uint32_t *spill_cnt, const uint32_t *spill_bin,
+                                                            uint32_t spill_cap, uint32_t *rsum_b)
+{
+    using U = typename KD::U;
+    __shared__ uint32_t s_off[GRP_MAX_RUNS + 1];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t part[4][256];
+    const uint32_t nr = *tb.nr;
+    const uint32_t tid = threadIdx.x, d2 = tid & 255u, q = tid >> 8;
+    const int shift = int(dev_shift(shift_dev, 0));
+    const U flip = U(flip64);
+    const bool nl = nan_last != 0;
+    const U *ykeys = static_cast<const U *>(ykeys_v);
+    const uint16_t *res16 = reinterpret_cast<const uint16_t *>(res);
+    uint32_t loaded = ~0u;  // the top digit whose run offsets s_off holds
+    for (uint32_t g = blockIdx.x; g < nr; g += gridDim.x) {
+        const uint32_t d1 = tb.bseg[g];
+        const uint64_t sb = tb.sbase[g];
+        const uint32_t a = uint32_t(tb.begin[g] - sb), b = uint32_t(tb.end[g] - sb);
+        __syncthreads();  // the previous range's readers of s_off / hist / part are done
+        if (d1 != loaded) {
+            for (uint32_t r = tid; r < ranges_a; r += 1024) s_off[r] = rpre_a[size_t(d1) * ranges_a + r];
+            if (tid == 0) s_off[ranges_a] = top_total[d1];
+            loaded = d1;
+        }
+        if (tid < 256) hist[tid] = 0;
+        __syncthreads();
+        // the runs holding positions a and b - 1: the last run starting at or before them
+        auto run_of = [&](uint32_t x) {
+            uint32_t lo = 0, hi = ranges_a;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (s_off[mid] <= x) lo = mid;
+                else hi = mid;
+            }
+            return lo;
+        };
+        const uint32_t ra = run_of(a), rb = run_of(b - 1);
+        auto count = [&](uint32_t lo, uint32_t hi) {
+            for (uint32_t x = lo + tid; x < hi; x += 1024)
+                atomicAdd(&hist[KD::digit(ykeys[sb + x], flip, nl, shift)], 1u);
+        };
+        if (ra == rb) {
+            count(a, b);
+        } else {
+            count(a, s_off[ra + 1]);
+            count(s_off[rb], b);
+        }
+        // the runs covered whole: their rows of pass A's histogram
+        const uint32_t bin = (d1 << 8) | d2;
+        uint32_t c = 0;
+        for (uint32_t r = ra + 1 + q; r < rb; r += 4) {
+            c += res16[size_t(r) * 65536 + bin];
+            const uint32_t ns = min(spill_cnt[r], spill_cap);
+            for (uint32_t i = 0; i < ns; ++i)
+                if (spill_bin[size_t(r) * spill_cap + i] == bin) c += H2_SPILL;
+        }
+        part[q][d2] = c;
+        __syncthreads();
+        if (q == 0) rsum_b[size_t(d2) * tb.cap + g] = hist[d2] + part[0][d2] + part[1][d2] + part[2][d2] + part[3][d2];
     }
 }
 
@@ -551,6 +775,7 @@ struct HybPlan {
     using SS = HybSegShape<KD, PL>;
     static constexpr uint64_t SEG_TILE = uint64_t(SS::BLOCK) * SS::ITEMS;
     // pass B's table must fit the workspace's [256][MAX_RANGES] range tables
+    // pass B's table must fit the workspace's [256][MAX_RANGES] range tables
     static uint64_t rk_b(size_t n)
     {
         const uint64_t per = uint64_t(MAX_RANGES - 256) * SEG_TILE;
@@ -561,6 +786,25 @@ struct HybPlan {
     static uint32_t lcap(size_t n) { return uint32_t(std::min<size_t>(HYB_BUCKETS, n / (cap(n) + 1) + 1)); }
     static uint32_t cap_d(size_t n) { return uint32_t((n + rk_d(n) - 1) / rk_d(n) + lcap(n)); }
     static size_t table_bytes(uint32_t cap) { return 3 * align256(size_t(cap) * 8) + 2 * align256(size_t(cap) * 4); }
+    // pass A: H2_RANGES ranges of whole tiles (its histogram read counts both MSD digits)
+    static uint32_t range_tiles_a(size_t n)
+    {
+        const size_t tiles = SortPlan<KD, PL>::tiles(n);
+        return uint32_t((tiles + H2_RANGES - 1) / H2_RANGES);
+    }
+    static uint32_t ranges_a(size_t n)
+    {
+        return uint32_t((SortPlan<KD, PL>::tiles(n) + range_tiles_a(n) - 1) / range_tiles_a(n));
+    }
+    static uint32_t spill_cap(size_t n)
+    {
+        return uint32_t(size_t(range_tiles_a(n)) * SortPlan<KD, PL>::tile(n) / H2_SPILL + 1);
+    }
+    static size_t h2_bytes(size_t n)
+    {
+        const size_t ra = ranges_a(n);
+        return align256(ra * 32768 * 4) + align256(ra * 4) + align256(ra * size_t(spill_cap(n)) * 4);
+    }
     // key / payload buffers: keys-only and key-value sorts ping-pong with one
     // scratch pair; argsort (read-only keys) runs between two scratch pairs
     static size_t data_bytes(size_t n)
@@ -573,7 +817,8 @@ struct HybPlan {
         return data_bytes(n) + align256(256 * 256 * 4) + align256(257 * 8) + 2 * align256(HYB_BUCKETS * 4) +
                align256(16) + table_bytes(cap_b(n)) + 2 * align256(size_t(lcap(n)) * 4) + table_bytes(cap_d(n)) +
                2 * align256(size_t(256) * cap_d(n) * 4) + align256(size_t(lcap(n)) * 256 * 4) +
-               align256(HYB_CTL_BYTES) + align256(size_t(lcap(n)) * 16) + align256((size_t(lcap(n)) + 1) * 4);
+               align256(HYB_CTL_BYTES) + align256(size_t(lcap(n)) * 16) + align256((size_t(lcap(n)) + 1) * 4) +
+               h2_bytes(n);
     }
 };
 
@@ -641,6 +886,11 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     h.seg_andor = reinterpret_cast<uint64_t *>(take(size_t(lcap) * 16));
     h.lrfirst = reinterpret_cast<uint32_t *>(take((size_t(lcap) + 1) * 4));
     h.plan = plan;
+    const uint32_t ranges_a = HP::ranges_a(n), rtiles_a = HP::range_tiles_a(n), spill_cap = HP::spill_cap(n);
+    uint32_t *h2_res = reinterpret_cast<uint32_t *>(take(size_t(ranges_a) * 32768 * 4));
+    uint32_t *h2_scnt = reinterpret_cast<uint32_t *>(take(size_t(ranges_a) * 4));
+    uint32_t *h2_sbin = reinterpret_cast<uint32_t *>(take(size_t(ranges_a) * spill_cap * 4));
+
     h.top_total = ws->total();
     h.segtot = segtot;
     h.nlarge = ctr;
@@ -650,37 +900,38 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     h.rk_d = HP::rk_d(n);
 
     // pass A: top digit over the whole array (keys -> Y; argsort: indices
-    // generated).  Its histogram read also takes the AND / OR of every order
-    // key, from which the plan kernel picks the digit pass A really sorts
-    // by (the top one unless all keys share it); another histogram is taken
-    // only in that case.
+    // generated).  Its histogram read counts the keys of each of its ranges
+    // by both MSD digits (pass B then needs no histogram read) and takes the
+    // AND / OR of every order key, from which the plan kernel picks the
+    // digit pass A really sorts by (the top one unless all keys share it);
+    // the histograms are taken again only in that case.
     {
         using Plan = SortPlan<KD, PL>;
-        const uint32_t rtiles = Plan::range_tiles(n), ranges = Plan::ranges(n);
-        UpArgs u{};
-        u.keys = keys; u.n = nn; u.range_keys = uint32_t(rtiles * Plan::tile(n)); u.shift = uint32_t(8 * (P - 1));
-        u.nan_last = nan_last; u.flip = flip; u.rsum = ws->rsum(); u.ranges = ranges; u.andor = andor; u.pass = 0;
-        u.andor_slots = ws->andor_slots();
+        H2Args u{};
+        u.keys = keys; u.n = nn; u.range_keys = uint32_t(rtiles_a * Plan::tile(n)); u.ranges = ranges_a;
+        u.shift_hi = uint32_t(8 * (P - 1)); u.flip = flip; u.nan_last = nan_last;
+        u.andor_slots = ws->andor_slots(); u.res = h2_res; u.spill_cnt = h2_scnt; u.spill_bin = h2_sbin;
+        u.spill_cap = spill_cap; u.rsum = ws->rsum();
         {
             KernelTimer t("upsweep", s);
-            hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(ranges), dim3(UP_BLOCK), 0, s, u);
+            hipLaunchKernelGGL((hyb_h2_kernel<KD>), dim3(ranges_a), dim3(H2_BLOCK), 0, s, u);
         }
         XSS_HIP_CHECK(hipGetLastError());
         {
             KernelTimer t("hyb_tables", s);
-            hipLaunchKernelGGL(hyb_plan_kernel<P>, dim3(1), dim3(256), 0, s, ws->andor_slots(), ranges, andor, plan);
+            hipLaunchKernelGGL(hyb_plan_kernel<P>, dim3(1), dim3(256), 0, s, ws->andor_slots(), ranges_a, andor,
+                               plan);
         }
         XSS_HIP_CHECK(hipGetLastError());
-        UpArgs u2 = u;  // the histogram of plan[0]'s digit, if it is not the top one
-        u2.andor = nullptr; u2.shift_dev = plan; u2.gate = plan + 3; u2.gate_min = 0; u2.nr_static = ranges;
+        H2Args u2 = u;  // the histograms of plan[0]'s digits, if they are not the top ones
+        u2.andor_slots = nullptr; u2.shift_dev = plan; u2.gate = plan + 3;
         {
             KernelTimer t("upsweep", s);
-            hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(std::min<uint32_t>(ranges, 512)), dim3(UP_BLOCK), 0, s,
-                               u2);
+            hipLaunchKernelGGL((hyb_h2_kernel<KD>), dim3(ranges_a), dim3(H2_BLOCK), 0, s, u2);
         }
         XSS_HIP_CHECK(hipGetLastError());
         ScanArgs sc{};
-        sc.rsum = ws->rsum(); sc.rpre = ws->rpre(); sc.total = ws->total(); sc.ranges = ranges;
+        sc.rsum = ws->rsum(); sc.rpre = ws->rpre(); sc.total = ws->total(); sc.ranges = ranges_a;
         {
             KernelTimer t("range_scan", s);
             hipLaunchKernelGGL(range_scan_kernel<0>, dim3(256), dim3(SCAN_BLOCK), 0, s, sc);
@@ -689,16 +940,26 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
         PassArgs a{};
         a.keys_in = keys; a.vals_in = vals; a.keys_out = yk; a.vals_out = yv; a.idx_out = nullptr;
         a.n = nn; a.shift = uint32_t(8 * (P - 1)); a.shift_dev = plan; a.flip = flip; a.nan_last = nan_last;
-        a.range_tiles = rtiles; a.ranges = ranges; a.rpre = ws->rpre(); a.total = ws->total();
+        a.range_tiles = rtiles_a; a.ranges = ranges_a; a.rpre = ws->rpre(); a.total = ws->total();
         a.first = true; a.last = false; a.canon_nan = false;
         const int rc = Plan::small_tiles(n)
-                           ? launch_downsweep<KD, PL, SmallPassShape<KD, PL>>(a, ranges, nan_last, true, s)
-                           : launch_downsweep<KD, PL, PassShape<KD, PL>>(a, ranges, nan_last, true, s);
+                           ? launch_downsweep<KD, PL, SmallPassShape<KD, PL>>(a, ranges_a, nan_last, true, s)
+                           : launch_downsweep<KD, PL, PassShape<KD, PL>>(a, ranges_a, nan_last, true, s);
         if (rc) return rc;
     }
     {
         KernelTimer t("hyb_tables", s);
         hipLaunchKernelGGL(hyb_top_ranges_kernel<0>, dim3(1), dim3(256), 0, s, h);
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    {
+        // pass B's digit counts: pass A's rows for the runs a range covers
+        // whole, the keys of the rest counted from Y
+        static_assert(H2_RANGES <= GRP_MAX_RUNS, "run offsets of a top bucket fit in LDS");
+        KernelTimer t("hyb_msd_upsweep", s);
+        hipLaunchKernelGGL((hyb_msd_hist_kernel<KD>), dim3(std::min<uint32_t>(cap_b, 2048)), dim3(1024), 0, s, h.tb,
+                           yk, flip, uint32_t(nan_last), plan + 1, ws->rpre(), ws->total(), ranges_a, h2_res, h2_scnt,
+                           h2_sbin, spill_cap, ws->rsum());
     }
     XSS_HIP_CHECK(hipGetLastError());
     uint32_t wnext = 0;  // next free work-stealing counter
@@ -707,30 +968,24 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
                         uint32_t *stot, int shift, const uint32_t *shift_dev, const uint32_t *gate,
                         uint32_t gate_min, const void *sk, const void *sv, void *dk, void *dv, bool last,
                         bool unstable, bool large, bool first_large) -> int {
-        // both kernels walk the table's ranges from a counter with a bounded
-        // grid: a table that turns out (nearly) empty costs few workgroups
-        // the large-bucket passes (usually near empty) take their ranges from
-        // counters with small grids; pass B (always full) one range per
-        // workgroup, as a counter's round trip per range costs it time
+        // The large-bucket passes (usually near empty) take their ranges from
+        // counters with small grids, so that a table that turns out (nearly)
+        // empty costs few workgroups; pass B (always full) takes one range
+        // per workgroup, and its digit counts come from pass A's two-digit
+        // histograms (hyb_group_hist_kernel), so it has no histogram read.
         uint32_t *up_ctr = large ? h.wctr + wnext++ : nullptr;
         uint32_t *down_ctr = large ? h.wctr + wnext++ : nullptr;
-        UpArgs u{};
-        u.keys = sk; u.n = nn; u.shift = uint32_t(shift); u.shift_dev = shift_dev; u.nan_last = nan_last;
-        u.flip = flip; u.rsum = rsum; u.ranges = t.cap; u.range_begin = t.begin; u.range_end = t.end;
-        u.nr_dev = t.nr; u.gate = gate; u.gate_min = gate_min; u.wctr = up_ctr; u.rbseg = t.bseg;
         if (large) {
+            UpArgs u{};
+            u.keys = sk; u.n = nn; u.shift = uint32_t(shift); u.shift_dev = shift_dev; u.nan_last = nan_last;
+            u.flip = flip; u.rsum = rsum; u.ranges = t.cap; u.range_begin = t.begin; u.range_end = t.end;
+            u.nr_dev = t.nr; u.gate = gate; u.gate_min = gate_min; u.wctr = up_ctr; u.rbseg = t.bseg;
             if (first_large) u.seg_andor_out = h.seg_andor;
             else u.seg_andor_in = h.seg_andor;
             u.seg_canon = canon_nan;
-        }
-        {
             KernelTimer kt(names[0], s);
-            if (large)
-                hipLaunchKernelGGL((upsweep_kernel<KD, true>), dim3(std::min<uint32_t>(t.cap, XSS_HYB_UP_GRID)),
-                                   dim3(UP_BLOCK), 0, s, u);
-            else
-                hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(std::min<uint32_t>(t.cap, MAX_RANGES)), dim3(UP_BLOCK),
-                                   0, s, u);
+            hipLaunchKernelGGL((upsweep_kernel<KD, true>), dim3(std::min<uint32_t>(t.cap, XSS_HYB_UP_GRID)),
+                               dim3(UP_BLOCK), 0, s, u);
         }
         XSS_HIP_CHECK(hipGetLastError());
         SegScanArgs sc{rsum, rpre, stot, t.head, t.bseg, 0, t.cap, t.nr, gate, gate_min, down_ctr};

@@ -552,19 +552,21 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         const uint32_t lp = lbase + i * 64;
-        const uint32_t d = (FULL || lp < valid) ? KD::digit(key[i], flip, NL, shift) : 255u;
-        if constexpr (UNST && FULL && PL != PL_IDX) {
+        const bool real = FULL || lp < valid;
+        const uint32_t d = real ? KD::digit(key[i], flip, NL, shift) : 255u;
+        if constexpr (UNST && PL != PL_IDX) {
             // keys only, or key-value (whose payloads inside a run of equal
             // keys may come in any order): each lane reserves its own slot
             // with one returning LDS add, no peer match.  A wave whose lanes share one digit (sorted
             // runs) reserves its 64 slots with one add by lane 0, since
             // same-address adds serialise; bit 24 marks it, and each lane's
             // slot is then lane 0's base + lane (resolved after the barrier).
+            // The padding of a partial tile takes no slot (bit 25).
             const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-            const bool uni = __ballot(d != d0) == 0;
+            const bool uni = __ballot(!real || d != d0) == 0;
             uint32_t old = 0;
-            if (!uni || lane == 0) old = atomicAdd(&wc[d], uni ? 64u : 1u);
-            rd[i] = old | (d << 16) | (uni ? (1u << 24) : 0u);
+            if (real && (!uni || lane == 0)) old = atomicAdd(&wc[d], uni ? 64u : 1u);
+            rd[i] = old | (d << 16) | (uni ? (1u << 24) : 0u) | (real ? 0u : (1u << 25));
             continue;
         }
         const uint64_t peers = match_any8(d);
@@ -605,8 +607,10 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
         uint32_t st = incl - tcount;
         for (uint32_t w = 0; w < wave; ++w) st += sm.wsum[w];
         sm.base[tid] = run - st;
-        // padding of a partial last tile was ranked as digit 255 (last)
-        run += tcount - ((!FULL && tid == 255) ? (TILE - valid) : 0u);
+        // padding of a partial last tile was ranked as digit 255 (last),
+        // except by the unstable ranking, where it takes no slot
+        constexpr bool PAD_SLOTS = !(UNST && PL != PL_IDX);
+        run += tcount - ((!FULL && PAD_SLOTS && tid == 255) ? (TILE - valid) : 0u);
 #pragma unroll
         for (int w = 0; w < WAVES; ++w) {
             const uint32_t c = sm.off[w * 256 + tid];
@@ -620,7 +624,8 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         uint32_t r = rd[i];
-        if constexpr (UNST && FULL && PL != PL_IDX) {
+        if constexpr (UNST && PL != PL_IDX) {
+            if (r >> 25) continue;  // padding
             if (r >> 24) r = (__builtin_amdgcn_readfirstlane(r) + lane) & 0xffffffu;
         }
         const uint32_t d = r >> 16;
@@ -759,7 +764,7 @@ __global__ __launch_bounds__(BLOCK) XSS_SCATTER_OCCUPANCY(KD, PL, BLOCK) void do
     for (; tile < nfull; ++tile)
         downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL, UNST>(b, sm, uint64_t(tile) * TILE, TILE, 0, run);
     for (; tile < tile1; ++tile)
-        downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(b, sm, uint64_t(tile) * TILE, b.n - tile * TILE, 0, run);
+        downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL, UNST>(b, sm, uint64_t(tile) * TILE, b.n - tile * TILE, 0, run);
 }
 
 // After a sort that skips constant digits: an odd number of moves left the
@@ -950,7 +955,7 @@ void seg_downsweep_kernel(PassArgs a, SegRanges r)
         uint64_t t = rb;
         for (; t + TILE <= re; t += TILE)
             downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL, UNST>(a, sm, t, TILE, sbase, run);
-        if (t < re) downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, t, uint32_t(re - t), sbase, run);
+        if (t < re) downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL, UNST>(a, sm, t, uint32_t(re - t), sbase, run);
     }
 }
 

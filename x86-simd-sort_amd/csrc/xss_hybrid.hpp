@@ -46,6 +46,10 @@ namespace xss {
 #ifndef XSS_HYB_WAVES
 #define XSS_HYB_WAVES 8
 #endif
+// 4-byte keys sorted in LDS as 12 unstable + 4 stable bits (1) or 8 + 8 (0)
+#ifndef XSS_HYB_SPLIT
+#define XSS_HYB_SPLIT 1
+#endif
 // waves per SIMD the local sort is compiled for (its SGPRs alone would cap it at 7)
 #define XSS_HYB_OCC __attribute__((amdgpu_waves_per_eu(XSS_HYB_WAVES)))
 constexpr int HYB_BLOCK = XSS_HYB_BLOCK;  // local sort workgroup (build-time knobs)
@@ -603,9 +607,13 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
     constexpr int WAVES = HYB_BLOCK / 64;
     constexpr int ITEMS = HybShape<KD, PL, SIZE>::ITEMS;
     constexpr uint32_t CAP = HybShape<KD, PL, SIZE>::CAP;
+    // 4-byte keys (not argsort): the 16 low bits sort as one unstable pass on
+    // 12 bits and one stable pass on 4 (SPLIT), so that only 4 bits pay the
+    // stable peer match
+    constexpr bool SPLIT = XSS_HYB_SPLIT && P == 4 && PL != PL_IDX;
     __shared__ U s_keys[CAP];
     __shared__ V s_vals[HAS_PL ? CAP : 1];
-    __shared__ uint32_t s_off[WAVES * 256];
+    __shared__ uint32_t s_off[SPLIT ? 4096 : WAVES * 256];
     __shared__ uint32_t s_wsum[WAVES];
     const uint32_t b = blockIdx.x;
     const uint32_t m = bsize[b];
@@ -732,7 +740,147 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
             }
         }
     };
-    if constexpr (P == 4) {
+    // SPLIT: pass 1 on the low 12 bits, unstable: every key takes its slot
+    // in its digit with one returning LDS add on a counter shared by the
+    // workgroup (4096 digits: few collisions), the counters are scanned, and
+    // the keys scattered.  Pass 2 on bits 12..15, stable, with per-thread
+    // counters (below).  Padding takes no slot in pass 1 and digit 15 (last)
+    // in pass 2.
+    auto split_12_4 = [&]() {
+        uint32_t *cnt = s_off;
+        for (uint32_t i = tid; i < 4096; i += HYB_BLOCK) cnt[i] = 0;
+        __syncthreads();
+        uint32_t rd[ITEMS];
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            if (uint32_t(i) < it) {
+                const uint32_t idx = lbase + i * 64;
+                const bool real = idx < m;
+                const uint32_t d = uint32_t(KD::ord(key[i], flip, NL)) & 0xfffu;
+                const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+                const bool uni = __ballot(!real || d != d0) == 0;
+                uint32_t old = 0;
+                if (real && (!uni || lane == 0)) old = atomicAdd(&cnt[d], uni ? 64u : 1u);
+                rd[i] = old | (d << 13) | (uni ? (1u << 25) : 0u) | (real ? 0u : (1u << 26));
+            }
+        }
+        __syncthreads();
+        // exclusive scan of the 4096 counters: 8 per thread, then across the workgroup
+        {
+            constexpr uint32_t PER = 4096 / HYB_BLOCK;
+            uint32_t v[PER], sum = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < PER; ++j) {
+                v[j] = cnt[tid * PER + j];
+                sum += v[j];
+            }
+            uint32_t incl = sum;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t t = __shfl_up(incl, off);
+                if (lane >= uint32_t(off)) incl += t;
+            }
+            if (lane == 63) s_wsum[wave] = incl;
+            __syncthreads();
+            uint32_t run = incl - sum;
+            for (uint32_t w = 0; w < wave; ++w) run += s_wsum[w];
+#pragma unroll
+            for (uint32_t j = 0; j < PER; ++j) {
+                cnt[tid * PER + j] = run;
+                run += v[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            if (uint32_t(i) < it) {
+                uint32_t r = rd[i];
+                if (r >> 26) continue;  // padding
+                if (r >> 25) r = (__builtin_amdgcn_readfirstlane(r) + lane) & 0x1ffffffu;
+                const uint32_t d = r >> 13;
+                const uint32_t slot = cnt[d] + (r & 0x1fffu);
+                s_keys[slot] = key[i];
+                if constexpr (HAS_PL) s_vals[slot] = val[i];
+            }
+        }
+        // pass 2: bits 12..15, stable.  Each thread takes `it` consecutive
+        // positions (blocked) and counts its keys per digit in registers
+        // (16 counters of 4 bits: at most 12 keys each); the per-thread
+        // counts, digit-major, are scanned in LDS; a key's slot is its
+        // (digit, thread) base plus its rank among the thread's keys.
+        uint16_t *cnt16 = reinterpret_cast<uint16_t *>(s_off);  // [16][HYB_BLOCK]
+        static_assert(16 * HYB_BLOCK * 2 <= 4096 * 4, "counters fit");
+        static_assert(ITEMS < 16, "4-bit counters");
+        U k2[ITEMS];
+        V v2[ITEMS];
+        uint32_t lr[ITEMS];
+        uint32_t c_lo = 0, c_hi = 0;
+        __syncthreads();  // pass 1's scatter is complete
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            if (uint32_t(i) < it) {
+                const uint32_t pos = tid * it + i;
+                const bool real = pos < m;
+                k2[i] = real ? s_keys[pos] : U(0);
+                if constexpr (HAS_PL) v2[i] = real ? s_vals[pos] : V(0);
+                const uint32_t d = real ? (uint32_t(KD::ord(k2[i], flip, NL)) >> 12) & 0xfu : 15u;
+                const uint32_t sh = (d & 7u) * 4u;
+                const uint32_t r = ((d < 8 ? c_lo : c_hi) >> sh) & 0xfu;
+                if (d < 8) c_lo += 1u << sh;
+                else c_hi += 1u << sh;
+                lr[i] = r | (d << 8);
+            }
+        }
+#pragma unroll
+        for (uint32_t d = 0; d < 16; ++d)
+            cnt16[d * HYB_BLOCK + tid] = uint16_t(((d < 8 ? c_lo : c_hi) >> ((d & 7u) * 4u)) & 0xfu);
+        __syncthreads();
+        {
+            constexpr uint32_t PER = 16;  // 16 * HYB_BLOCK counters, HYB_BLOCK threads
+            uint32_t v[PER], sum = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < PER; ++j) {
+                v[j] = cnt16[tid * PER + j];
+                sum += v[j];
+            }
+            uint32_t incl = sum;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t t = __shfl_up(incl, off);
+                if (lane >= uint32_t(off)) incl += t;
+            }
+            if (lane == 63) s_wsum[wave] = incl;
+            __syncthreads();
+            uint32_t run = incl - sum;
+            for (uint32_t w = 0; w < wave; ++w) run += s_wsum[w];
+#pragma unroll
+            for (uint32_t j = 0; j < PER; ++j) {
+                cnt16[tid * PER + j] = uint16_t(run);
+                run += v[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            if (uint32_t(i) < it) {
+                const uint32_t d = lr[i] >> 8;
+                const uint32_t slot = cnt16[d * HYB_BLOCK + tid] + (lr[i] & 0xffu);
+                s_keys[slot] = k2[i];
+                if constexpr (HAS_PL) s_vals[slot] = v2[i];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            if (uint32_t(i) < it) {
+                key[i] = s_keys[lbase + i * 64];
+                if constexpr (HAS_PL) val[i] = s_vals[lbase + i * 64];
+            }
+        }
+    };
+    if constexpr (SPLIT) {
+        split_12_4();
+    } else if constexpr (P == 4) {
         // 4-byte keys: both passes inline (measured 8% faster than the loop)
         pass(0, true);
         pass(8, false);

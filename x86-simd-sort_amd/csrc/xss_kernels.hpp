@@ -492,6 +492,12 @@ struct PassArgs {
     uint32_t canon_if_last;
 };
 
+// 1: the scatter keeps each sorted slot's digit in an LDS byte array; 0:
+// the write-out recomputes it from the key (one LDS write and read fewer
+// per key; build-time knob)
+#ifndef XSS_DIG_LDS
+#define XSS_DIG_LDS 0
+#endif
 template <class KD, int PL, int BLOCK, int ITEMS>
 struct DownsweepSmem {
     using U = typename KD::U;
@@ -502,7 +508,9 @@ struct DownsweepSmem {
     uint64_t base[256];          // global position of tile slot 0 of each digit
     uint32_t wsum[WAVES < 4 ? 4 : WAVES];
     U keys[TILE];                // the tile, sorted by digit
+#if XSS_DIG_LDS
     uint8_t dig[TILE];           // digit of each sorted slot
+#endif
     V vals[PL == PL_NONE ? 1 : TILE];
 };
 
@@ -631,7 +639,9 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
         const uint32_t d = r >> 16;
         const uint32_t lpos = wc[d] + (r & 0xffffu);
         sm.keys[lpos] = key[i];
+#if XSS_DIG_LDS
         sm.dig[lpos] = uint8_t(d);
+#endif
         if constexpr (HAS_PL) sm.vals[lpos] = val[i];
     }
     __syncthreads();
@@ -643,7 +653,13 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
     for (int i = 0; i < ITEMS; ++i) {
         const uint32_t lpos = i * BLOCK + tid;
         if (FULL || lpos < valid) {
+            const U x = sm.keys[lpos];
+#if XSS_DIG_LDS
             const uint64_t g = sm.base[sm.dig[lpos]] + lpos;
+#else
+            // the slot's digit, recomputed from its key (no LDS byte array)
+            const uint64_t g = sm.base[KD::digit(x, flip, NL, shift)] + lpos;
+#endif
             if constexpr (PL == PL_IDX) {
                 if (a.last) {
                     if (a.idx32_out) a.idx32_out[g] = uint32_t(sm.vals[lpos] - uint32_t(seg_base));
@@ -651,7 +667,6 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
                     continue;
                 }
             }
-            const U x = sm.keys[lpos];
             if constexpr (KD::KIND == Kind::Float && NL) {
                 kout[g] = (a.canon_nan && KD::is_nan(x)) ? KD::QNAN : x;
             } else {

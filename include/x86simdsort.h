@@ -14,7 +14,9 @@
  * Types: qsort/qselect/partial_qsort/argsort/argselect for uint16_t,
  * int16_t, _Float16 (where the compiler has it), float, double, uint32_t,
  * int32_t, uint64_t, int64_t; keyvalue_* for the 36 pairs of
- * {uint32_t, int32_t, float, uint64_t, int64_t, double}.
+ * {uint32_t, int32_t, float, uint64_t, int64_t, double}, and for uint16_t,
+ * int16_t and _Float16 keys with those six value types (an addition: the
+ * reference has no 16-bit key-value sort).
  */
 #ifndef XSS_AMD_X86SIMDSORT_H
 #define XSS_AMD_X86SIMDSORT_H

@@ -20,7 +20,10 @@
  *   xss_gpu_keyvalue_partial_sort_<K>_<V> x86simdsort::keyvalue_partial_sort<K,V> lib/x86simdsort.h:63-70, lib/x86simdsort.cpp:234-246
  *
  * <T> in {u16, i16, f16, u32, i32, f32, u64, i64, f64}; <K>,<V> in
- * {u32, i32, f32, u64, i64, f64} (the 36 pairs of lib/x86simdsort.cpp:280-293).
+ * {u32, i32, f32, u64, i64, f64} (the 36 pairs of lib/x86simdsort.cpp:280-293),
+ * and <K> in {u16, i16, f16} with the same six <V>: 16-bit keys with
+ * payloads, which the reference does not provide (it sorts 16-bit keys
+ * only alone or for argsort, lib/x86simdsort.cpp:167-204).
  * argsort/argselect write n size_t (uint64_t) indices to `arg` and leave the
  * keys untouched, like the reference (src/x86simdsort-static-incl.h:92-121).
  *
@@ -163,6 +166,9 @@ XSS_DECLARE_ARGFORM(f64, double)
     XSS_DECLARE_KV(ktag, K, i64, int64_t)  \
     XSS_DECLARE_KV(ktag, K, f64, double)
 
+XSS_DECLARE_KV_ALLV(u16, uint16_t)
+XSS_DECLARE_KV_ALLV(i16, int16_t)
+XSS_DECLARE_KV_ALLV(f16, xss_f16_bits)
 XSS_DECLARE_KV_ALLV(u32, uint32_t)
 XSS_DECLARE_KV_ALLV(i32, int32_t)
 XSS_DECLARE_KV_ALLV(f32, float)

@@ -45,7 +45,7 @@ def test_cpp_templates_resolve_to_library_symbols(tmp_path):
     for t in types:
         src += [f"{{ {t} *a = nullptr; x86simdsort::qsort(a, 0); x86simdsort::qselect(a, 0, 0);",
                 f"  x86simdsort::partial_qsort(a, 0, 0); x86simdsort::argsort(a, 0); x86simdsort::argselect(a, 0, 0); }}"]
-    for k in kv:
+    for k in kv + ["uint16_t", "int16_t"]:  # 16-bit keys: beyond the reference's 36 pairs
         for v in kv:
             src += [f"{{ {k} *a = nullptr; {v} *b = nullptr; x86simdsort::keyvalue_qsort(a, b, 0);",
                     f"  x86simdsort::keyvalue_select(a, b, 0, 0); x86simdsort::keyvalue_partial_sort(a, b, 0, 0); }}"]
@@ -56,7 +56,7 @@ def test_cpp_templates_resolve_to_library_symbols(tmp_path):
     subprocess.run(["g++", "-std=c++17", "-c", "-I", INC, str(f), "-o", str(obj)], check=True)
     und = subprocess.run(["nm", "-u", str(obj)], capture_output=True, text=True, check=True).stdout
     need = {ln.split()[-1] for ln in und.splitlines() if "x86simdsort" in ln}
-    assert len(need) == 8 * 5 + 36 * 3
+    assert len(need) == 8 * 5 + (36 + 12) * 3
     missing = sorted(need - exported())
     assert not missing, missing
 

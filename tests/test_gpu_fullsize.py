@@ -124,14 +124,20 @@ def kv_input(kind, n):
         g = torch.Generator(device="cuda")
         g.manual_seed(12)
         return "f32", "u32", torch.randn(n, device="cuda", generator=g).view(torch.int32)
+    if kind == "f16_nan":   # 16-bit keys with payloads (no reference counterpart)
+        return "f16", "u32", D.gen("f16", "nan", n, 14)
+    if kind == "i16_dups":
+        return "i16", "u64", D.gen("i16", "dups", n, 15)
     tag = "u64" if kind.endswith("64") else "f32"
     return tag, "u64" if tag == "u64" else "u32", D.gen(tag, kind.split("_")[0], n, 13)
 
 
-KV_KINDS = ["zipf", "normal", "dups_64", "sorted_64", "dups_32", "sorted_32"]
+KV_KINDS = ["zipf", "normal", "dups_64", "sorted_64", "dups_32", "sorted_32", "f16_nan", "i16_dups"]
 
 
 def kv_n(kind):
+    if kind in ("f16_nan", "i16_dups"):
+        return 1 << 29
     return 1 << 27 if kind in ("zipf", "dups_64", "sorted_64") else 1 << 28
 
 
@@ -146,9 +152,10 @@ def test_keyvalue_qsort_full(xss, kind, desc):
     gk, gv = k.clone(), v.clone()
     what = f"keyvalue_qsort<{ktag},{vtag}> {kind} n={n} desc={desc}"
     print(what, flush=True)
-    xss.keyvalue_qsort(darray(xss, gk, ktag), darray(xss, gv, vtag), False, desc)
+    hasnan = kind.endswith("nan")
+    xss.keyvalue_qsort(darray(xss, gk, ktag), darray(xss, gv, vtag), hasnan, desc)
     D.check_gather(gk, k, gv.to(torch.int64), what)
-    D.check_sort(gk, k, ktag, False, desc, False, what)
+    D.check_sort(gk, k, ktag, hasnan, desc, False, what)
 
 
 @pytest.mark.parametrize("kind", ["zipf", "normal"])

@@ -64,7 +64,8 @@ def test_argselect(xss, tag):
         assert np.array_equal(T.bits(d.numpy()), T.bits(a)), "argselect modified its input"
 
 
-KV = [(k, v) for k in T.KV_TAGS for v in T.KV_TAGS]  # the 36 pairs (lib/x86simdsort.cpp:280-293)
+# the 36 pairs (lib/x86simdsort.cpp:280-293), then 16-bit keys with the six value types
+KV = [(k, v) for k in T.KV_TAGS + T.KV16_TAGS for v in T.KV_TAGS]
 KV_DISTS = ["random", "dups", "constant", "signed_zeros", "nan", "rand_with_max_and_nan", "sorted"]
 
 

@@ -58,7 +58,8 @@ def test_argsort(xss, tag, desc, sizes):
         assert np.array_equal(T.bits(d.numpy()), T.bits(a))
 
 
-KV_PAIRS = [(k, v) for k in T.KV_TAGS for v in T.KV_TAGS]
+# the 36 pairs of lib/x86simdsort.cpp:280-293, then 16-bit keys with the six value types
+KV_PAIRS = [(k, v) for k in T.KV_TAGS + T.KV16_TAGS for v in T.KV_TAGS]
 
 
 @pytest.mark.parametrize("desc", [False, True])

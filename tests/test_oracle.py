@@ -188,6 +188,25 @@ def test_oracle_kv_matches_reference(ktag):
 
 
 @needs_ref
+@pytest.mark.parametrize("ktag", T.KV16_TAGS)
+def test_oracle_kv16_matches_reference_argsort(ktag):
+    """16-bit keys with payloads have no reference entry point; the oracle's
+    kv sort of them must equal the reference's argsort applied to keys and
+    values (payloads compared as multisets per run of equal keys)."""
+    O, R = T.ORACLE(), T.REF()
+    for vtag in T.KV_TAGS:
+        for seed, (dist, n) in enumerate((d, n) for d in ["random", "dups", "nan_payloads", "signed_zeros"]
+                                         for n in [2, 100, 3000]):
+            k = T.gen(ktag, dist, n, 43000 + seed)
+            v = T.gen(vtag, "random", n, 44000 + seed)
+            hasnan = dist in T.NAN_DISTS
+            for desc in (False, True):
+                idx = R.argsort(k, ktag, hasnan, desc).astype(np.int64)
+                ok, ov = O.keyvalue_qsort(k, v, ktag, vtag, hasnan, desc)
+                T.assert_kv_equal(k[idx], v[idx], ok, ov, ktag, f"kv16 {ktag} {vtag} {dist} {n}", zeros_as_values=True)
+
+
+@needs_ref
 def test_oracle_object_sort_matches_reference():
     import ctypes
     O, R = T.oracle(), T.ref()

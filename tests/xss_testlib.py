@@ -31,6 +31,7 @@ TAGS = {
 UINT = {2: np.uint16, 4: np.uint32, 8: np.uint64}
 FLOAT_TAGS = ("f16", "f32", "f64")
 KV_TAGS = ["u32", "i32", "f32", "u64", "i64", "f64"]
+KV16_TAGS = ["u16", "i16", "f16"]  # 16-bit kv keys: beyond the reference's 36 pairs
 # canonical NaN written by qsort(hasnan) (src/xss-custom-float.h:34-38,81-85)
 QNAN = {"f16": 0x7C01, "f32": 0x7FC00000, "f64": 0x7FF8000000000000}
 

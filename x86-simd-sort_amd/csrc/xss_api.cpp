@@ -222,7 +222,7 @@ XSS_HOST_T(u64, uint64_t, uint64_t)
 XSS_HOST_T(i64, int64_t, int64_t)
 XSS_HOST_T(f64, double, double)
 
-#define XSS_HOST_KV(ktag, K, vtag, V)                                                                   \
+#define XSS_HOST_KV(ktag, K, CK, vtag, V)                                                                \
     template <>                                                                                         \
     void keyvalue_qsort<K, V>(K * key, V * val, size_t n, bool hasnan, bool descending)                 \
     {                                                                                                   \
@@ -232,7 +232,7 @@ XSS_HOST_T(f64, double, double)
         char *dv = d + align256(kb);                                                                    \
         h2d(d, key, kb);                                                                                \
         h2d(dv, val, vb);                                                                               \
-        xss_ok(xss_gpu_keyvalue_qsort_##ktag##_##vtag(reinterpret_cast<K *>(d), reinterpret_cast<V *>(dv), n, \
+        xss_ok(xss_gpu_keyvalue_qsort_##ktag##_##vtag(reinterpret_cast<CK *>(d), reinterpret_cast<V *>(dv), n, \
                                                       hasnan, descending, stream()),                    \
                "keyvalue_qsort");                                                                       \
         d2h(key, d, kb);                                                                                \
@@ -248,7 +248,7 @@ XSS_HOST_T(f64, double, double)
         char *dv = d + align256(kb);                                                                    \
         h2d(d, key, kb);                                                                                \
         h2d(dv, val, vb);                                                                               \
-        xss_ok(xss_gpu_keyvalue_select_##ktag##_##vtag(reinterpret_cast<K *>(d), reinterpret_cast<V *>(dv), k, \
+        xss_ok(xss_gpu_keyvalue_select_##ktag##_##vtag(reinterpret_cast<CK *>(d), reinterpret_cast<V *>(dv), k, \
                                                        n, hasnan, descending, stream()),                \
                "keyvalue_select");                                                                      \
         d2h(key, d, kb);                                                                                \
@@ -264,7 +264,7 @@ XSS_HOST_T(f64, double, double)
         char *dv = d + align256(kb);                                                                    \
         h2d(d, key, kb);                                                                                \
         h2d(dv, val, vb);                                                                               \
-        xss_ok(xss_gpu_keyvalue_partial_sort_##ktag##_##vtag(reinterpret_cast<K *>(d),                  \
+        xss_ok(xss_gpu_keyvalue_partial_sort_##ktag##_##vtag(reinterpret_cast<CK *>(d),                  \
                                                              reinterpret_cast<V *>(dv), k, n, hasnan,   \
                                                              descending, stream()),                     \
                "keyvalue_partial_sort");                                                                \
@@ -273,20 +273,23 @@ XSS_HOST_T(f64, double, double)
         sync();                                                                                         \
     }
 
-#define XSS_HOST_KV_ALLV(ktag, K)       \
-    XSS_HOST_KV(ktag, K, u32, uint32_t) \
-    XSS_HOST_KV(ktag, K, i32, int32_t)  \
-    XSS_HOST_KV(ktag, K, f32, float)    \
-    XSS_HOST_KV(ktag, K, u64, uint64_t) \
-    XSS_HOST_KV(ktag, K, i64, int64_t)  \
-    XSS_HOST_KV(ktag, K, f64, double)
+#define XSS_HOST_KV_ALLV(ktag, K, CK)       \
+    XSS_HOST_KV(ktag, K, CK, u32, uint32_t) \
+    XSS_HOST_KV(ktag, K, CK, i32, int32_t)  \
+    XSS_HOST_KV(ktag, K, CK, f32, float)    \
+    XSS_HOST_KV(ktag, K, CK, u64, uint64_t) \
+    XSS_HOST_KV(ktag, K, CK, i64, int64_t)  \
+    XSS_HOST_KV(ktag, K, CK, f64, double)
 
-XSS_HOST_KV_ALLV(u32, uint32_t)
-XSS_HOST_KV_ALLV(i32, int32_t)
-XSS_HOST_KV_ALLV(f32, float)
-XSS_HOST_KV_ALLV(u64, uint64_t)
-XSS_HOST_KV_ALLV(i64, int64_t)
-XSS_HOST_KV_ALLV(f64, double)
+XSS_HOST_KV_ALLV(u16, uint16_t, uint16_t)
+XSS_HOST_KV_ALLV(i16, int16_t, int16_t)
+XSS_HOST_KV_ALLV(f16, _Float16, xss_f16_bits)
+XSS_HOST_KV_ALLV(u32, uint32_t, uint32_t)
+XSS_HOST_KV_ALLV(i32, int32_t, int32_t)
+XSS_HOST_KV_ALLV(f32, float, float)
+XSS_HOST_KV_ALLV(u64, uint64_t, uint64_t)
+XSS_HOST_KV_ALLV(i64, int64_t, int64_t)
+XSS_HOST_KV_ALLV(f64, double, double)
 
 namespace detail {
 #define XSS_HOST_OBJ(tag, K)                                                                            \

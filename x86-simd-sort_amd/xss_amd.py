@@ -25,6 +25,8 @@ TAGS = {
     "u64": np.uint64, "i64": np.int64, "f64": np.float64,
 }
 KV_TAGS = ["u32", "i32", "f32", "u64", "i64", "f64"]
+# keyvalue_* key types: the reference's six, and 16-bit keys (no reference counterpart)
+KV_KEY_TAGS = ["u16", "i16", "f16"] + KV_TAGS
 TAG_OF = {np.dtype(v): k for k, v in TAGS.items()}
 
 XSS_OK = 0

@@ -28,16 +28,25 @@ The JSON line carries:
                       present there
   pipeline            the whole sort against the roofline at the algorithmic
                       (1 + 2P) * key bytes per key of an 8-bit LSD sort; the
-                      MSD-first design moves 8 * key bytes per key (two
-                      passes of histogram read + read + write, then one read
-                      and one write); per-kernel ms per sort; and the same
-                      sort timed on the LSD path (four passes) for reference
+                      MSD-first design moves about 7 * key bytes per key (one
+                      two-digit histogram read, two passes of read + write,
+                      then one read and one write of each bucket); per-kernel
+                      ms per sort; and the same sort timed on the LSD path
+                      (four passes) for reference
   segmented           the batch form: 8192 uneven float32 arrays (~775M keys)
                       sorted by one graph-captured plan against a loop of
                       plain per-array calls on the same GPU
+  serving             the served workload (serve_bench.py) at 10k and 50k
+                      requests/s: p50 / p99 latency, keys/s, launch groups
+                      and coalesced requests of the server and of one plain
+                      call at a time
   cpu_baseline        the reference library (oracle/_ref, AVX-512 dispatch +
-                      OpenMP, <= 16 threads) timed on this host on a bounded
-                      sample of the same distribution
+                      OpenMP, <= 16 threads) timed on this host: `value` on
+                      the headline array itself (2^28 keys), `rows` on
+                      bounded samples of the other workloads (1M-key qsort,
+                      argsort, qselect, Zipf key-value sort, object_qsort,
+                      the segmented arrays as a loop of calls), each beside
+                      the GPU's rate on the full workload
 """
 import argparse
 import ctypes
@@ -74,33 +83,170 @@ def lib_timings(L):
     return {names[i].decode(): (ms[i], cnt[i]) for i in range(max(0, k))}
 
 
-def cpu_baseline(dtype, seconds=12.0):
-    """The reference itself (kind "reference") when oracle/_ref is built,
-    else the oracle's C restatement (kind "port", one thread)."""
-    import numpy as np
+def _ref_or_port():
+    """(library, prefix, kind, threads, isa): the reference itself when
+    oracle/_ref is built, else the oracle's C restatement (one thread)."""
     import xss_testlib as T
-    n = 1 << 25
-    a = T.gen(dtype, "random", n, 99)
     L = T.ref()
     if L is not None:
         threads = min(16, os.cpu_count() or 1)
         L.xssref_set_threads(threads)
-        fn, kind, isa = getattr(L, f"xssref_qsort_{dtype}"), "reference", L.xssref_isa().decode()
-    else:
-        L = T.oracle()
-        threads = 1
-        fn, kind, isa = getattr(L, f"xss_oracle_qsort_{dtype}"), "port", "scalar C"
+        return L, "xssref_", "reference", threads, L.xssref_isa().decode()
+    return T.oracle(), "xss_oracle_", "port", 1, "scalar C"
+
+
+def _cpu_time(fn, make, reps, seconds):
+    """Median seconds of fn(buf) over up to `reps` runs (at least 2) within
+    `seconds`; make() builds each run's input outside the timed call."""
     times = []
     t_end = time.time() + seconds
-    while len(times) < 3 or (time.time() < t_end and len(times) < 50):
-        b = a.copy()
+    while len(times) < 2 or (len(times) < reps and time.time() < t_end):
+        buf = make()
         t0 = time.perf_counter()
-        fn(ctypes.c_void_p(b.ctypes.data), ctypes.c_size_t(n), 0, 0)
+        fn(buf)
         times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    return {"value": round(n / med / 1e9, 4), "unit": "Gkeys/s", "cores": threads, "kind": kind,
-            "sample": f"qsort<{dtype}> of 2^25 uniform full-range keys, median of {len(times)} sorts "
-                      f"(input restore excluded); dispatch {isa}"}
+    return statistics.median(times), len(times)
+
+
+def _gpu_ms(torch, restore, call, reps=5):
+    """Median ms of call() between HIP events on the current stream."""
+    ts = []
+    for _ in range(reps + 1):
+        restore()
+        s = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        rc = call()
+        e1.record(s)
+        torch.cuda.synchronize()
+        assert rc in (0, None), rc
+        ts.append(e0.elapsed_time(e1))
+    return statistics.median(ts[1:])
+
+
+def _zipf_u64(n, seed):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    z = rng.zipf(1.1, n).astype(np.uint64)
+    bad = z >= (1 << 32)
+    while bad.any():
+        z[bad] = rng.zipf(1.1, int(bad.sum())).astype(np.uint64)
+        bad = z >= (1 << 32)
+    return z
+
+
+def cpu_baseline(torch, L, dtype, gpu_ms_headline, seg=None, quick=False):
+    """The reference (kind "reference": oracle/_ref, its AVX-512 dispatch and
+    OpenMP team, <= 16 threads) timed on this host, the input of every run
+    built outside the timed call as the reference's benchmarks do
+    (benchmarks/bench-qsort.hpp:12-16).  `value` is the headline workload,
+    qsort of 2^28 uniform keys, on the full array; `rows` time the other
+    workloads on bounded samples and give the GPU's rate on the full
+    workload beside them."""
+    import numpy as np
+    import xss_testlib as T
+    R, pre, kind, threads, isa = _ref_or_port()
+    p = ctypes.c_void_p
+    z = ctypes.c_size_t
+    n = 1 << 24 if quick else 1 << 28
+    a = T.gen(dtype, "random", n, 99)
+    f = getattr(R, f"{pre}qsort_{dtype}")
+    sec, runs = _cpu_time(lambda b: f(p(b.ctypes.data), z(n), 0, 0), a.copy, 3, 20.0)
+    out = {"value": round(n / sec / 1e9, 4), "unit": "Gkeys/s", "cores": threads, "kind": kind,
+           "sample": f"qsort<{dtype}> of 2^{n.bit_length() - 1} uniform full-range keys (the headline array), "
+                     f"median of {runs} sorts; dispatch {isa}",
+           "gpu_over_cpu": round((float(1 << 28) / (gpu_ms_headline * 1e-3)) / (n / sec), 1)}
+    del a
+    if kind != "reference":
+        out["rows"] = "the oracle port times the headline only"
+        return out
+    rows = []
+    cur = lambda: ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)  # noqa: E731
+
+    def row(workload, cpu_n, cpu_s, cpu_runs, gpu_n, gpu_ms, unit_note=""):
+        c = cpu_n / cpu_s / 1e9
+        g = gpu_n / (gpu_ms * 1e-3) / 1e9
+        rows.append({"workload": workload, "cpu_sample_n": cpu_n, "cpu_runs": cpu_runs, "cpu_Gkeys_s": round(c, 4),
+                     "gpu_n": gpu_n, "gpu_ms": round(gpu_ms, 4), "gpu_Gkeys_s": round(g, 3),
+                     "gpu_over_cpu": round(g / c, 1), **({"note": unit_note} if unit_note else {})})
+
+    # C1: qsort<float> of 1,000,000 keys U(-1e6, 1e6)
+    m = 1_000_000
+    c1 = (np.random.default_rng(1).uniform(-1e6, 1e6, m)).astype(np.float32)
+    sec, runs = _cpu_time(lambda b: R.xssref_qsort_f32(p(b.ctypes.data), z(m), 0, 0), c1.copy, 30, 3.0)
+    src = torch.from_numpy(c1).cuda()
+    w = torch.empty_like(src)
+    gms = _gpu_ms(torch, lambda: w.copy_(src),
+                  lambda: L.xss_gpu_qsort_f32(p(w.data_ptr()), z(m), 0, 0, cur()), 20)
+    row("qsort<float>, 1,000,000 keys U(-1e6, 1e6)", m, sec, runs, m, gms)
+    # argsort<float>, uniform bits
+    cn, gn = (1 << 22, 1 << 24) if quick else (1 << 25, 1 << 28)
+    ca = T.gen("f32", "random", cn, 5)
+    idx = np.zeros(cn, np.uint64)
+    sec, runs = _cpu_time(lambda b: R.xssref_argsort_f32(p(b.ctypes.data), z(cn), 0, 0, p(idx.ctypes.data)),
+                          lambda: ca, 3, 10.0)
+    gsrc = device_input(torch, "f32", gn, 6)
+    gidx = torch.empty(gn, dtype=torch.int64, device="cuda")
+    gms = _gpu_ms(torch, lambda: None, lambda: L.xss_gpu_argsort_f32(p(gsrc.data_ptr()), p(gidx.data_ptr()), z(gn),
+                                                                     0, 0, cur()))
+    row(f"argsort<float>, uniform full-range keys (GPU 2^{gn.bit_length() - 1})", cn, sec, runs, gn, gms)
+    # qselect<float> at k = n/2
+    sec, runs = _cpu_time(lambda b: R.xssref_qselect_f32(p(b.ctypes.data), z(cn // 2), z(cn), 0, 0), ca.copy, 5, 5.0)
+    w = torch.empty_like(gsrc)
+    gms = _gpu_ms(torch, lambda: w.copy_(gsrc),
+                  lambda: L.xss_gpu_qselect_f32(p(w.data_ptr()), z(gn // 2), z(gn), 0, 0, cur()))
+    row(f"qselect<float> at k = n/2, uniform full-range keys (GPU 2^{gn.bit_length() - 1})", cn, sec, runs, gn, gms)
+    del gsrc, gidx, w, ca, idx
+    torch.cuda.empty_cache()
+    # keyvalue_qsort<uint64_t, uint64_t>, Zipf(1.1) keys below 2^32, values = index
+    cn, gn = (1 << 22, 1 << 23) if quick else (1 << 25, 1 << 27)
+    zk = _zipf_u64(cn, 3)
+    zv = np.arange(cn, dtype=np.uint64)
+    vb = zv.copy()
+
+    def kv_make():
+        vb[:] = zv
+        return zk.copy()
+    sec, runs = _cpu_time(lambda b: R.xssref_keyvalue_qsort_u64_u64(p(b.ctypes.data), p(vb.ctypes.data), z(cn), 0, 0),
+                          kv_make, 3, 10.0)
+    gk0 = torch.from_numpy(_zipf_u64(gn, 4).view(np.int64)).cuda()
+    gv0 = torch.arange(gn, dtype=torch.int64, device="cuda")
+    gk, gv = torch.empty_like(gk0), torch.empty_like(gv0)
+    gms = _gpu_ms(torch, lambda: (gk.copy_(gk0), gv.copy_(gv0)),
+                  lambda: L.xss_gpu_keyvalue_qsort_u64_u64(p(gk.data_ptr()), p(gv.data_ptr()), z(gn), 0, 0, cur()))
+    row(f"keyvalue_qsort<uint64_t, uint64_t>, Zipf(1.1) keys < 2^32 (GPU 2^{gn.bit_length() - 1})",
+        cn, sec, runs, gn, gms)
+    del gk0, gv0, gk, gv, zk, zv, vb
+    torch.cuda.empty_cache()
+    # object_qsort of {double x, y, z} by the Euclidean norm
+    on = 1 << 20 if quick else 1 << 24
+    xyz = np.random.default_rng(7).uniform(-1, 1, (on, 3))
+    sec, runs = _cpu_time(lambda b: R.xssref_object_qsort_xyz(p(b.ctypes.data), z(on)), xyz.copy, 2, 8.0)
+    gx0 = torch.from_numpy(xyz.reshape(-1)).cuda()
+    gx = torch.empty_like(gx0)
+    keys = torch.empty(on, dtype=torch.float64, device="cuda")
+    gms = _gpu_ms(torch, lambda: gx.copy_(gx0),
+                  lambda: L.xss_gpu_object_qsort_xyz_norm(p(gx.data_ptr()), z(on), p(keys.data_ptr()), cur()))
+    row(f"object_qsort of 2^{on.bit_length() - 1} {{x, y, z}} doubles by the Euclidean norm", on, sec, runs, on, gms,
+        "the reference publishes 2.208 s at 16M objects (misc/bench-simdobjsort.txt)")
+    del gx0, gx, keys, xyz
+    torch.cuda.empty_cache()
+    # the segmented workload as a loop of x86simdsort::qsort calls over its first 1024 arrays
+    if seg is not None:
+        sizes, vals_host = seg["sizes"][:1024], seg["host_sample"]
+        off = np.concatenate([[0], np.cumsum(sizes)])
+        cn = int(off[-1])
+
+        def seg_loop(b):
+            base = b.ctypes.data
+            for i in range(len(sizes)):
+                R.xssref_qsort_f32(p(base + 4 * int(off[i])), z(int(sizes[i])), 1, 0)
+        sec, runs = _cpu_time(seg_loop, vals_host.copy, 3, 8.0)
+        row("8192 float arrays (len log-uniform [16, 2^20], 0.1% NaN): x86simdsort::qsort per array "
+            "(CPU: the first 1024 arrays); GPU: one segmented plan, graph replay",
+            cn, sec, runs, seg["n_keys"], seg["segmented_ms"])
+    out["rows"] = rows
+    return out
 
 
 def segmented_batch(torch, xss_amd, L, tuning, reps=3):
@@ -199,9 +345,13 @@ def segmented_batch(torch, xss_amd, L, tuning, reps=3):
     L.xss_gpu_segplan_destroy(p)
     out.update({"argsort_i32_ms": round(t_arg, 3), "argsort_i32_loop_ms": round(t_arg_loop, 3),
                 "qselect_ms": round(t_sel, 3), "qselect_loop_ms": round(t_sel_loop, 3)})
+    # the CPU loop's sample: the first 1024 arrays as the plan sorted them
+    m = int(off[1024])
+    extra = {"sizes": sizes, "n_keys": n, "segmented_ms": t_graph,
+             "host_sample": pristine[:m].cpu().numpy().view(np.float32)}
     del work, pristine, ints, idx, idx64
     torch.cuda.empty_cache()
-    return out
+    return out, extra
 
 
 def traffic_from_profiles(kernel, dtype, n):
@@ -228,6 +378,8 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 28)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-seg", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--quick-cpu", action="store_true", help="smaller CPU samples (a quick check of the legs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X, and the sort of one array "
@@ -292,8 +444,10 @@ def main():
     # launch: scatter-type kernels read and write every key once, histogram
     # kernels read every key once (the hybrid's per-bucket LDS sort also
     # reads and writes each key once).
+    # (hyb_msd_upsweep, pass B's counts, reads only the ~5% of keys in runs
+    # that its ranges cut, so it is no candidate)
     alg_per_launch = {"downsweep": 2 * n * key_bytes, "hyb_msd_downsweep": 2 * n * key_bytes,
-                      "hyb_local": 2 * n * key_bytes, "upsweep": n * key_bytes, "hyb_msd_upsweep": n * key_bytes}
+                      "hyb_local": 2 * n * key_bytes, "upsweep": n * key_bytes}
     dom = max((k for k in alg_per_launch if k in kt), key=lambda k: kt[k][0])
     dom_ms, dom_cnt = kt[dom]
     dom_avg_ms = dom_ms / max(1, dom_cnt)
@@ -303,17 +457,19 @@ def main():
     hybrid = "hyb_local" in kt
     # algorithmic minimum of an 8-bit LSD sort: one histogram read + P x (read + write)
     pipe_bytes = n * (1 + 2 * P) * key_bytes
-    # what this design moves: MSD-first = 2 x (histogram read + read + write)
-    # + the LDS bucket sort's read + write (buckets too large for LDS add
-    # segmented passes); LSD = P x (histogram read + read + write)
-    moved_bytes = n * (8 if hybrid else 3 * P) * key_bytes
+    # what this design moves: MSD-first = one histogram read, two passes of
+    # read + write and the LDS bucket sort's read + write (7 x key bytes;
+    # pass B's counts read ~5% more, buckets too large for LDS add segmented
+    # passes); LSD = P x (histogram read + read + write)
+    moved_bytes = n * (7 if hybrid else 3 * P) * key_bytes
     pipe_gbs = pipe_bytes / (ms_per_step / 1e3) / 1e9
     gkeys = n / (ms_per_step / 1e3) / 1e9
-    kernel_desc = {"downsweep": "downsweep_kernel (LSD scatter pass, top digit)",
-                   "hyb_msd_downsweep": "seg_downsweep_kernel (second-digit pass inside top buckets)",
-                   "hyb_local": "hyb_local_kernel (LDS radix of each 16-bit bucket, low 16 bits)",
-                   "upsweep": "upsweep_kernel (digit histograms)",
-                   "hyb_msd_upsweep": "upsweep_kernel (second-digit histograms)"}[dom]
+    kernel_desc = {"downsweep": ("downsweep_kernel (pass A: scatter on the top varying digit)" if hybrid
+                                 else "downsweep_kernel (LSD scatter pass)"),
+                   "hyb_msd_downsweep": "seg_downsweep_kernel (pass B: second digit inside each top bucket)",
+                   "hyb_local": "hyb_local_kernel (LDS sort of each 16-bit bucket by its low 16 bits)",
+                   "upsweep": ("hyb_h2_kernel (two-digit histogram read)" if hybrid
+                               else "upsweep_kernel (digit histograms)")}[dom]
     # the same sort on the LSD path (four 8-bit passes), for reference
     lsd_ms = None
     if hybrid:
@@ -358,15 +514,27 @@ def main():
         "wall_s_timed_loop_incl_restore": round(wall, 4),
         "sorted_ok": ok,
     }
+    seg_extra = None
     if not args.no_seg:
         tuning = {}
         tp = os.path.join(REPO, "x86-simd-sort_amd", "tuning.json")
         if os.path.exists(tp):
             tuning = json.load(open(tp)).get("segmented_f32", {})
-        line["segmented"] = segmented_batch(torch, xss_amd, L, tuning)
+        line["segmented"], seg_extra = segmented_batch(torch, xss_amd, L, tuning)
         line["segmented_speedup_vs_loop"] = line["segmented"]["speedup_vs_loop"]
+    if not args.no_serve:
+        from serve_bench import serving_leg
+        legs = serving_leg(torch)
+        line["serving"] = {
+            "workload": "open-loop Poisson traces of 1000 requests: 70% qsort<float> 1K-16M keys, 20% "
+                        "keyvalue_qsort<u64,u64> Zipf keys 1K-4M, 10% qselect/partial_qsort<u32> 1K-16M; "
+                        "server: 4 streams, coalescing up to 2^20 keys; baseline: one plain call at a time",
+            "rates": [{"rate_req_s": r["rate_req_s"], "total_keys": r["total_keys"],
+                       **{f"{m}_{k}": r[m][k] for m in ("server", "plain_call_baseline")
+                          for k in ("p50_ms", "p99_ms", "keys_per_s", "launch_groups", "coalesced_requests")}}
+                      for r in legs]}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(dt)
+        line["cpu_baseline"] = cpu_baseline(torch, L, dt, ms_per_step, seg_extra, args.quick_cpu)
     print(json.dumps(line))
 
 

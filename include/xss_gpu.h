@@ -241,14 +241,19 @@ XSS_API int xss_gpu_loop_qselect_f32(float *keys, const uint64_t *offsets, size_
  * Serving form (no counterpart in the reference): a continuous-batching
  * request loop.  A worker thread owns `nstreams` HIP streams, admits queued
  * requests in arrival order while a stream is free and the admitted HBM
- * footprint (keys + values + scratch) stays within budget_bytes, coalesces
- * queued qsort requests of at most coalesce_max keys (and short enough for
- * one LDS workgroup) with the same key type and flags into one launch, and
- * runs every other request through its entry point above on its own stream,
- * so requests overlap on the device.  reserve_bytes grows each stream's
- * pooled workspace up front.  one_at_a_time = 1 serves requests strictly
- * one by one (the plain-call baseline).  Requests work in place on the
- * caller's device buffers, which must stay valid until the request is done.
+ * footprint (keys + values + scratch) stays within budget_bytes, and
+ * coalesces queued qsort / keyvalue_qsort requests of at most coalesce_max
+ * keys (capped at 2^24) with the same op, key type, value width and flags
+ * into one launch group: a pointer-list LDS sort for the short ones and
+ * shared segmented passes (gathered into a per-stream arena and scattered
+ * back) for the others, up to 2^24 keys and 1024 requests per group.  Every
+ * other request runs through its entry point above on its own stream, so
+ * requests overlap on the device.  coalesce_max = 0 turns coalescing off;
+ * otherwise each stream reserves an arena of 512 MiB (outside budget_bytes)
+ * at creation.  reserve_bytes grows each stream's pooled workspace up
+ * front.  one_at_a_time = 1 serves requests strictly one by one (the
+ * plain-call baseline).  Requests work in place on the caller's device
+ * buffers, which must stay valid until the request is done.
  */
 typedef struct xss_request {
     int op;            /* 0 qsort, 1 keyvalue_qsort, 2 qselect, 3 partial_qsort */

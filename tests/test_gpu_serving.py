@@ -2,10 +2,12 @@
 its result when run alone through the plain call.
 
 A seeded mix of qsort (all key types, short and long, NaN / descending
-flags), keyvalue_qsort, qselect and partial_qsort requests is served three
-ways -- a burst into 4 streams with coalescing, a 2-stream server whose
-admission budget holds only a few requests at a time, and one request at a
-time -- and compared with the same requests run alone:
+flags), keyvalue_qsort, qselect and partial_qsort requests is served four
+ways -- a burst into 4 streams with coalescing of requests up to 2^20 keys
+(short ones by the pointer-list LDS sort, longer ones by shared segmented
+passes), the same with coalescing of LDS-sized ones only, a 2-stream server
+whose admission budget holds only a few requests at a time, and one request
+at a time -- and compared with the same requests run alone:
   * qsort: bit-exact (a keys-only result is fixed by its keys, and
     qsort(hasnan) writes the canonical NaN);
   * keyvalue_qsort: keys bit-exact, payloads as a multiset within each run
@@ -51,7 +53,8 @@ def _mix(count, seed):
             n = int(np.exp(rng.uniform(np.log(4097), np.log(1 << 18))))
             op, vtag, k = "qsort", None, 0
         elif u < 0.8:
-            tag = T.KV_TAGS[int(rng.integers(0, len(T.KV_TAGS)))]
+            kt = T.KV_TAGS + T.KV16_TAGS
+            tag = kt[int(rng.integers(0, len(kt)))]
             vtag = KV_VAL[int(rng.integers(0, len(KV_VAL)))]
             n = int(np.exp(rng.uniform(np.log(2), np.log(1 << 17))))
             op, k = "keyvalue_qsort", 0
@@ -155,17 +158,59 @@ def mix_and_alone(xss):
 
 def test_serving_burst_coalesced(xss, mix_and_alone):
     mix, alone = mix_and_alone
-    served, stats = _serve(xss, mix, budget_bytes=64 << 30, streams=4, coalesce_max=4096)
+    served, stats = _serve(xss, mix, budget_bytes=64 << 30, streams=4, coalesce_max=1 << 20)
     _compare(mix, alone, served, "burst")
     assert stats["completed"] == len(mix)
-    assert stats["coalesced"] > 0, stats  # short qsorts queued together shared a launch
+    assert stats["coalesced"] > 0, stats  # requests queued together shared launch groups
     assert stats["launch_groups"] < len(mix)
+
+
+def test_serving_burst_short_only(xss, mix_and_alone):
+    """Coalescing limited to what one LDS workgroup sorts (pointer-list form)."""
+    mix, alone = mix_and_alone
+    served, stats = _serve(xss, mix, budget_bytes=64 << 30, streams=4, coalesce_max=4096)
+    _compare(mix, alone, served, "burst short")
+    assert stats["completed"] == len(mix) and stats["coalesced"] > 0, stats
+
+
+def _medium_mix(count, seed):
+    """Medium requests only (longer than an LDS sort): qsort<f32> with NaNs
+    and qsort<u16> descending of 4097..2^20 keys, keyvalue_qsort<u64,u64>
+    and <f16,u32> of 2049..2^19 keys."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        c = i % 4
+        if c < 2:
+            tag, hasnan, desc = ("f32", True, False) if c == 0 else ("u16", False, True)
+            n = int(np.exp(rng.uniform(np.log(4097), np.log(1 << 20))))
+            keys = T.gen(tag, "nan" if c == 0 else "dups", n, 500 + i)
+            out.append(("qsort", tag, None, keys, None, 0, hasnan, desc))
+        else:
+            tag, vtag = ("u64", "u64") if c == 2 else ("f16", "u32")
+            n = int(np.exp(rng.uniform(np.log(2049), np.log(1 << 19))))
+            keys = T.gen(tag, "dups" if c == 2 else "nan_payloads", n, 600 + i)
+            vals = rng.integers(0, 2 ** 31, n).astype(T.TAGS[vtag])
+            out.append(("keyvalue_qsort", tag, vtag, keys, vals, 0, c == 3, bool(i % 8 == 7)))
+    return out
+
+
+def test_serving_medium_groups(xss):
+    """Medium requests queued together share gathered segmented passes: every
+    result equals the plain call's and the oracle's."""
+    mix = _medium_mix(48, 91)
+    alone = [_alone(xss, it) for it in mix]
+    served, stats = _serve(xss, mix, budget_bytes=64 << 30, streams=2, coalesce_max=1 << 20)
+    _compare(mix, alone, served, "medium groups")
+    assert stats["completed"] == len(mix)
+    assert stats["coalesced"] >= len(mix) // 2, stats
+    assert stats["launch_groups"] <= len(mix) // 2, stats
 
 
 def test_serving_tight_budget(xss, mix_and_alone):
     mix, alone = mix_and_alone
     # 8 MiB admits a few requests at a time; larger ones run alone when nothing else is in flight
-    served, stats = _serve(xss, mix, budget_bytes=8 << 20, streams=2, coalesce_max=1024)
+    served, stats = _serve(xss, mix, budget_bytes=8 << 20, streams=2, coalesce_max=1 << 20)
     _compare(mix, alone, served, "tight budget")
     assert stats["completed"] == len(mix)
 

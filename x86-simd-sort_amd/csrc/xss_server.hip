@@ -9,13 +9,20 @@
 //     releases their HBM footprint and stream;
 //   * admits queued requests in arrival order while a stream is free and the
 //     admitted footprint (keys + values + scratch) stays within the budget;
-//   * coalesces queued qsort requests short enough for one LDS workgroup
-//     (same key type and flags) into one launch of the pointer-list block
-//     sort; every other request runs through its C-ABI entry point on its
-//     stream, so up to S requests overlap on the device while new ones keep
-//     arriving and are admitted as streams and memory free up.
-// Each stream keeps the library's pooled workspace, reserved up front, so
-// steady-state requests allocate nothing.
+//   * coalesces queued qsort and keyvalue_qsort requests of at most
+//     coalesce_max keys with the same key type, value width and flags into
+//     one launch group: those short enough for one LDS workgroup are sorted
+//     in place by one launch of the pointer-list block sort; the others are
+//     gathered into the stream's arena as segments of one buffer, sorted by
+//     one set of segmented LSD passes (upsweep -> per-segment scan ->
+//     downsweep per digit, over a range table built on the host and uploaded
+//     with one copy) and scattered back;
+//   * runs every other request through its C-ABI entry point on its stream,
+//     so up to S requests overlap on the device while new ones keep arriving
+//     and are admitted as streams and memory free up.
+// Each stream keeps the library's pooled workspace, reserved up front, and a
+// fixed arena for coalesced groups, so steady-state requests allocate
+// nothing.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -29,7 +36,17 @@
 using namespace xss;
 using Clock = std::chrono::steady_clock;
 
+constexpr uint32_t MAX_GROUP = 1024;  // requests per coalesced launch group
+
 namespace {
+
+// Coalesced groups: medium requests (longer than one LDS workgroup sorts)
+// share segmented passes over up to GROUP_KEYS keys gathered into the
+// stream's arena, cut into ranges of 1..MAX_RANGE_TILES downsweep tiles
+// (tiles are at least 4096 keys, so GROUP_RANGES bounds the range table).
+constexpr uint64_t GROUP_KEYS = uint64_t(1) << 24;
+constexpr uint32_t MAX_RANGE_TILES = 8;
+constexpr uint32_t GROUP_RANGES = uint32_t(GROUP_KEYS / 4096u) + MAX_GROUP;
 
 enum Op : int { OP_QSORT = 0, OP_KV_QSORT = 1, OP_QSELECT = 2, OP_PARTIAL = 3 };
 // key/value type codes: index into this list
@@ -101,14 +118,18 @@ int launch_one(const xss_request &r, hipStream_t s)
 #define XSS_KV(tag, K)                                                                                      \
     return v64 ? xss_gpu_keyvalue_qsort_##tag##_u64((K *)r.keys, (uint64_t *)r.vals, r.n, r.hasnan, r.descending, s) \
                : xss_gpu_keyvalue_qsort_##tag##_u32((K *)r.keys, (uint32_t *)r.vals, r.n, r.hasnan, r.descending, s)
+        if (r.val_type < 3 || r.val_type >= NTAGS)
+            return set_error(XSS_ERR_INVALID, "keyvalue_qsort has no %s values", TAG[r.val_type < 0 ? 0 : r.val_type % NTAGS]);
         switch (kt) {
+        case 0: XSS_KV(u16, uint16_t);
+        case 1: XSS_KV(i16, int16_t);
+        case 2: XSS_KV(f16, xss_f16_bits);
         case 3: XSS_KV(u32, uint32_t);
         case 4: XSS_KV(i32, int32_t);
         case 5: XSS_KV(f32, float);
         case 6: XSS_KV(u64, uint64_t);
         case 7: XSS_KV(i64, int64_t);
-        case 8: XSS_KV(f64, double);
-        default: return set_error(XSS_ERR_INVALID, "keyvalue_qsort has no %s keys", TAG[kt]);
+        default: XSS_KV(f64, double);
         }
 #undef XSS_KV
     }
@@ -116,46 +137,210 @@ int launch_one(const xss_request &r, hipStream_t s)
     return set_error(XSS_ERR_INVALID, "bad op %d", r.op);
 }
 
-// one launch sorting many short arrays (pointer-list block sort)
-template <class KD>
-int launch_small_group(const uint64_t *d_ptrs, const uint32_t *d_len, uint32_t count, bool hasnan, bool desc,
-                       hipStream_t s)
+// ---------------------------------------------------------------- groups
+
+// Device addresses of one group's tables, packed by the host into the
+// stream's arena (the same offsets in its pinned mirror).
+struct GroupDev {
+    const uint64_t *sm_k, *sm_v;   // [nsmall] small requests: keys / values pointers
+    const uint32_t *sm_len;        // [nsmall]
+    const uint64_t *md_k, *md_v;   // [nmed] medium requests: keys / values pointers
+    const uint64_t *r_begin, *r_end, *r_sbase;  // [nr] ranges of the staging buffer
+    const uint32_t *r_bseg, *r_head;            // [nr] medium index, first range of one
+    uint32_t *rsum, *rpre, *segtot;             // pass counts
+    char *kA, *kB, *vA, *vB;                    // staging: keys and values, ping-pong
+    uint32_t nsmall, nmed, nr;
+    uint64_t nkeys;                             // medium keys in the staging buffer
+};
+
+// Medium requests <-> the staging buffer: one workgroup per range (a range
+// lies inside one request).  back = 0 gathers, 1 scatters.
+template <class T>
+__global__ __launch_bounds__(256) void group_copy_kernel(const uint64_t *r_begin, const uint64_t *r_end,
+                                                         const uint64_t *r_sbase, const uint32_t *r_bseg,
+                                                         const uint64_t *ptrs, T *stage, int back)
 {
-    using BS = BlockShape<KD, PL_NONE>;
-    BlockSortArgs b{};
-    b.arr_keys = d_ptrs;
-    b.arr_len = d_len;
-    b.nan_last = (KD::KIND == Kind::Float) && hasnan;
-    b.flip = desc ? ~0ull : 0ull;
-    b.canon_nan = b.nan_last;
-    KernelTimer t("serve_small", s);
-    hipLaunchKernelGGL((block_sort_kernel<KD, PL_NONE, BS::BLOCK, BS::MAXN>), dim3(count), dim3(BS::BLOCK), 0, s, b);
+    const uint32_t r = blockIdx.x;
+    const uint64_t b = r_begin[r], e = r_end[r], sb = r_sbase[r];
+    T *user = reinterpret_cast<T *>(ptrs[r_bseg[r]]) - sb;
+    if (back)
+        for (uint64_t x = b + threadIdx.x; x < e; x += 256) user[x] = stage[x];
+    else
+        for (uint64_t x = b + threadIdx.x; x < e; x += 256) stage[x] = user[x];
+}
+
+template <class T>
+int group_copy(const GroupDev &g, const uint64_t *ptrs, char *stage, int back, hipStream_t s)
+{
+    KernelTimer t(back ? "serve_scatter" : "serve_gather", s);
+    hipLaunchKernelGGL(group_copy_kernel<T>, dim3(g.nr), dim3(256), 0, s, g.r_begin, g.r_end, g.r_sbase, g.r_bseg,
+                       ptrs, reinterpret_cast<T *>(stage), back);
     XSS_HIP_CHECK(hipGetLastError());
     return XSS_OK;
 }
 
-int small_group(int kt, const uint64_t *d_ptrs, const uint32_t *d_len, uint32_t count, bool hasnan, bool desc,
-                hipStream_t s)
+// One coalesced group of keys-only (PL_NONE) or key-value (PL_V32 / PL_V64)
+// sorts: the pointer-list block sort for the small requests, then the
+// medium ones gathered, sorted by P segmented LSD passes (P is even, so the
+// result is back in kA / vA) and scattered.  qsort(hasnan) writes the
+// canonical NaN, as the plain call does; key-value sorts keep NaN bits.
+template <class KD, int PL>
+int run_group(const GroupDev &g, bool hasnan, bool desc, hipStream_t s)
+{
+    using U = typename KD::U;
+    using V = typename PayloadT<PL>::type;
+    using BS = BlockShape<KD, PL>;
+    using PS = SegShape<KD, PL>;
+    constexpr int P = KD::PASSES;
+    static_assert(P % 2 == 0, "an even pass count ends in the gathered buffer");
+    constexpr bool HAS_VAL = PL != PL_NONE;
+    const bool nan_last = (KD::KIND == Kind::Float) && hasnan;
+    const bool canon = nan_last && !HAS_VAL;
+    const uint64_t flip = desc ? ~0ull : 0ull;
+    if (g.nsmall) {
+        BlockSortArgs b{};
+        b.arr_keys = g.sm_k;
+        b.arr_vals = HAS_VAL ? g.sm_v : nullptr;
+        b.arr_len = g.sm_len;
+        b.nan_last = nan_last;
+        b.flip = flip;
+        b.canon_nan = canon;
+        KernelTimer t("serve_small", s);
+        hipLaunchKernelGGL((block_sort_kernel<KD, PL, BS::BLOCK, BS::MAXN>), dim3(g.nsmall), dim3(BS::BLOCK), 0, s, b);
+        XSS_HIP_CHECK(hipGetLastError());
+    }
+    if (!g.nr) return XSS_OK;
+    int rc = group_copy<U>(g, g.md_k, g.kA, 0, s);
+    if (!rc && HAS_VAL) rc = group_copy<V>(g, g.md_v, g.vA, 0, s);
+    if (rc) return rc;
+    const void *src_k = g.kA, *src_v = g.vA;
+    for (int pass = 0; pass < P; ++pass) {
+        UpArgs u{};
+        u.keys = src_k;
+        u.n = uint32_t(g.nkeys);
+        u.shift = 8 * pass;
+        u.nan_last = nan_last;
+        u.flip = flip;
+        u.rsum = g.rsum;
+        u.ranges = g.nr;
+        u.range_begin = g.r_begin;
+        u.range_end = g.r_end;
+        {
+            KernelTimer t("serve_upsweep", s);
+            hipLaunchKernelGGL((upsweep_kernel<KD>), dim3(g.nr), dim3(UP_BLOCK), 0, s, u);
+        }
+        XSS_HIP_CHECK(hipGetLastError());
+        SegScanArgs sc{g.rsum, g.rpre, g.segtot, g.r_head, g.r_bseg, g.nr, g.nr, nullptr};
+        {
+            KernelTimer t("serve_scan", s);
+            hipLaunchKernelGGL(seg_scan_kernel<0>, dim3(256), dim3(256), 0, s, sc);
+        }
+        XSS_HIP_CHECK(hipGetLastError());
+        PassArgs a{};
+        a.keys_in = src_k;
+        a.vals_in = src_v;
+        a.keys_out = (pass % 2 == 0) ? g.kB : g.kA;
+        a.vals_out = (pass % 2 == 0) ? g.vB : g.vA;
+        a.n = uint32_t(g.nkeys);
+        a.shift = 8 * pass;
+        a.flip = flip;
+        a.nan_last = nan_last;
+        a.first = pass == 0;
+        a.last = pass == P - 1;
+        a.canon_nan = (pass == P - 1) && canon;
+        SegRanges r{g.r_begin, g.r_end, g.r_sbase, g.r_bseg, g.rpre, g.segtot, g.nr, g.nr, nullptr};
+        {
+            KernelTimer t("serve_downsweep", s);
+            if (nan_last)
+                hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, true>), dim3(g.nr),
+                                   dim3(PS::BLOCK), 0, s, a, r);
+            else
+                hipLaunchKernelGGL((seg_downsweep_kernel<KD, PL, PS::BLOCK, PS::ITEMS, false>), dim3(g.nr),
+                                   dim3(PS::BLOCK), 0, s, a, r);
+        }
+        XSS_HIP_CHECK(hipGetLastError());
+        src_k = a.keys_out;
+        src_v = a.vals_out;
+    }
+    rc = group_copy<U>(g, g.md_k, g.kA, 1, s);
+    if (!rc && HAS_VAL) rc = group_copy<V>(g, g.md_v, g.vA, 1, s);
+    return rc;
+}
+
+template <class KD>
+int run_group_pl(int val_bytes, const GroupDev &g, bool hasnan, bool desc, hipStream_t s)
+{
+    if (val_bytes == 0) return run_group<KD, PL_NONE>(g, hasnan, desc, s);
+    if (val_bytes == 4) return run_group<KD, PL_V32>(g, hasnan, desc, s);
+    return run_group<KD, PL_V64>(g, hasnan, desc, s);
+}
+
+int run_group_typed(int kt, int val_bytes, const GroupDev &g, bool hasnan, bool desc, hipStream_t s)
 {
     switch (kt) {
-    case 0: return launch_small_group<TypeKey<uint16_t>::KD>(d_ptrs, d_len, count, hasnan, desc, s);
-    case 1: return launch_small_group<TypeKey<int16_t>::KD>(d_ptrs, d_len, count, hasnan, desc, s);
-    case 2: return launch_small_group<TypeKey<f16_tag>::KD>(d_ptrs, d_len, count, hasnan, desc, s);
-    case 3: return launch_small_group<TypeKey<uint32_t>::KD>(d_ptrs, d_len, count, hasnan, desc, s);
-    case 4: return launch_small_group<TypeKey<int32_t>::KD>(d_ptrs, d_len, count, hasnan, desc, s);
-    case 5: return launch_small_group<TypeKey<float>::KD>(d_ptrs, d_len, count, hasnan, desc, s);
-    case 6: return launch_small_group<TypeKey<uint64_t>::KD>(d_ptrs, d_len, count, hasnan, desc, s);
-    case 7: return launch_small_group<TypeKey<int64_t>::KD>(d_ptrs, d_len, count, hasnan, desc, s);
-    default: return launch_small_group<TypeKey<double>::KD>(d_ptrs, d_len, count, hasnan, desc, s);
+    case 0: return run_group_pl<TypeKey<uint16_t>::KD>(val_bytes, g, hasnan, desc, s);
+    case 1: return run_group_pl<TypeKey<int16_t>::KD>(val_bytes, g, hasnan, desc, s);
+    case 2: return run_group_pl<TypeKey<f16_tag>::KD>(val_bytes, g, hasnan, desc, s);
+    case 3: return run_group_pl<TypeKey<uint32_t>::KD>(val_bytes, g, hasnan, desc, s);
+    case 4: return run_group_pl<TypeKey<int32_t>::KD>(val_bytes, g, hasnan, desc, s);
+    case 5: return run_group_pl<TypeKey<float>::KD>(val_bytes, g, hasnan, desc, s);
+    case 6: return run_group_pl<TypeKey<uint64_t>::KD>(val_bytes, g, hasnan, desc, s);
+    case 7: return run_group_pl<TypeKey<int64_t>::KD>(val_bytes, g, hasnan, desc, s);
+    default: return run_group_pl<TypeKey<double>::KD>(val_bytes, g, hasnan, desc, s);
     }
 }
 
-// the largest request one LDS workgroup sorts, per key type (keys only)
-uint32_t small_limit(int kt) { return tag_bytes(kt) == 8 ? 2048 : 4096; }
+// largest request the pointer-list block sort takes (BlockShape's MAXN)
+template <class KD>
+uint32_t small_max_pl(int val_bytes)
+{
+    if (val_bytes == 0) return BlockShape<KD, PL_NONE>::MAXN;
+    if (val_bytes == 4) return BlockShape<KD, PL_V32>::MAXN;
+    return BlockShape<KD, PL_V64>::MAXN;
+}
+
+uint32_t small_limit(int kt, int val_bytes)
+{
+    switch (tag_bytes(kt)) {
+    case 2: return small_max_pl<TypeKey<uint16_t>::KD>(val_bytes);
+    case 4: return small_max_pl<TypeKey<uint32_t>::KD>(val_bytes);
+    default: return small_max_pl<TypeKey<uint64_t>::KD>(val_bytes);
+    }
+}
+
+// keys per range of the segmented passes: whole tiles of SegShape, as many
+// per range as keep about 1024 workgroups per pass (at most
+// MAX_RANGE_TILES), so a group of a few requests still fills the GPU
+uint32_t range_keys(int kt, int val_bytes, uint64_t med_keys)
+{
+    const int bytes = int(tag_bytes(kt)) + val_bytes;
+    const uint32_t tile = bytes == 4 ? uint32_t(XSS_SEG_BLOCK_C1) * XSS_SEG_ITEMS_C1 : 512u * (bytes > 8 ? 8 : 16);
+    const uint64_t want = (med_keys + uint64_t(tile) * 1024 - 1) / (uint64_t(tile) * 1024);
+    return tile * uint32_t(std::min<uint64_t>(std::max<uint64_t>(want, 1), MAX_RANGE_TILES));
+}
+
+// value bytes of a request that may join a group (0: keys only), or -1
+int group_val_bytes(const xss_request &r)
+{
+    if (r.key_type < 0 || r.key_type >= NTAGS) return -1;
+    if (r.op == OP_QSORT) return 0;
+    if (r.op == OP_KV_QSORT && r.val_type >= 3 && r.val_type < NTAGS) return int(tag_bytes(r.val_type));
+    return -1;
+}
+
+// Per-stream memory of coalesced groups: tables (device + pinned mirror),
+// pass counts and the staging buffers.
+struct Arena {
+    char *h_tab = nullptr, *d_tab = nullptr;
+    uint32_t *rsum = nullptr, *rpre = nullptr, *segtot = nullptr;
+    char *stage = nullptr;   // kA | kB | vA | vB, GROUP_KEYS * 8 bytes each
+    static size_t table_bytes()
+    {
+        return size_t(MAX_GROUP) * (8 + 8 + 4 + 8 + 8) + size_t(GROUP_RANGES) * (8 * 3 + 4 * 2) + 16 * 256;
+    }
+};
 
 }  // namespace
-
-constexpr uint32_t MAX_GROUP = 1024;  // requests per coalesced launch
 
 struct xss_server {
     size_t budget = 0, used = 0;
@@ -164,10 +349,7 @@ struct xss_server {
     int device = 0;
     std::vector<hipStream_t> streams;
     std::vector<bool> busy;
-    std::vector<uint64_t *> h_ptrs;   // pinned staging of the pointer lists, per stream
-    std::vector<uint32_t *> h_len;
-    std::vector<uint64_t *> d_ptrs;   // device copies, per stream
-    std::vector<uint32_t *> d_len;
+    std::vector<Arena> arenas;        // per stream (when coalescing)
     std::deque<Req *> queue;
     std::vector<Group *> inflight;
     std::mutex mu;
@@ -181,7 +363,112 @@ struct xss_server {
     void loop();
     bool retire();
     bool admit();
+    int launch_group(int s, int val_bytes, Group *g);
 };
+
+// Take the queued requests that can share the head's group (same op, key
+// type, value width and flags, at most coalesce_max keys, medium keys within
+// GROUP_KEYS, footprint within the budget) off the queue, pack their tables
+// into the stream's pinned mirror, upload them with one copy and launch.
+int xss_server::launch_group(int s, int vb, Group *g)
+{
+    const xss_request &fr = *queue.front()->r;
+    const uint32_t smax = small_limit(fr.key_type, vb);
+    const uint64_t cap = std::min<uint64_t>(coalesce_max, GROUP_KEYS);
+    std::vector<Req *> sm, md;
+    uint64_t med_keys = 0;
+    for (auto it = queue.begin(); it != queue.end() && sm.size() + md.size() < MAX_GROUP;) {
+        const xss_request &q = *(*it)->r;
+        const bool match = q.op == fr.op && q.key_type == fr.key_type && group_val_bytes(q) == vb &&
+                           q.hasnan == fr.hasnan && q.descending == fr.descending && q.n <= cap;
+        const bool small = q.n <= smax;
+        // the head is admitted whatever its size (the caller checked the budget)
+        const bool fits = g->reqs.empty() || used + g->footprint + (*it)->footprint <= budget;
+        if (match && fits && (small || med_keys + q.n <= GROUP_KEYS)) {
+            (small ? sm : md).push_back(*it);
+            if (!small) med_keys += q.n;
+            g->reqs.push_back(*it);
+            g->footprint += (*it)->footprint;
+            it = queue.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    if (g->reqs.size() == 1) {  // nothing to share a launch with: the plain call, without the copies
+        g->footprint = g->reqs[0]->footprint;
+        return launch_one(fr, streams[s]);
+    }
+    const uint32_t rk = range_keys(fr.key_type, vb, med_keys);
+    size_t nr = 0;
+    for (Req *r : md) nr += (r->r->n + rk - 1) / rk;
+    if (nr > GROUP_RANGES) return set_error(XSS_ERR_INVALID, "group range table overflow (%zu ranges)", nr);
+    // pack the tables
+    Arena &A = arenas[s];
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 15) & ~size_t(15);
+        return o;
+    };
+    const size_t nsm = sm.size(), nmd = md.size();
+    const size_t o_smk = take(nsm * 8), o_smv = take(nsm * 8), o_sml = take(nsm * 4);
+    const size_t o_mdk = take(nmd * 8), o_mdv = take(nmd * 8);
+    const size_t o_rb = take(nr * 8), o_re = take(nr * 8), o_rs = take(nr * 8);
+    const size_t o_rbs = take(nr * 4), o_rh = take(nr * 4);
+    if (off > Arena::table_bytes()) return set_error(XSS_ERR_INVALID, "group tables exceed the arena");
+    char *h = A.h_tab;
+    for (size_t i = 0; i < nsm; ++i) {
+        const xss_request &q = *sm[i]->r;
+        reinterpret_cast<uint64_t *>(h + o_smk)[i] = reinterpret_cast<uint64_t>(q.keys);
+        reinterpret_cast<uint64_t *>(h + o_smv)[i] = reinterpret_cast<uint64_t>(q.vals);
+        reinterpret_cast<uint32_t *>(h + o_sml)[i] = uint32_t(q.n);
+    }
+    uint64_t base = 0;
+    size_t r = 0;
+    for (size_t i = 0; i < nmd; ++i) {
+        const xss_request &q = *md[i]->r;
+        reinterpret_cast<uint64_t *>(h + o_mdk)[i] = reinterpret_cast<uint64_t>(q.keys);
+        reinterpret_cast<uint64_t *>(h + o_mdv)[i] = reinterpret_cast<uint64_t>(q.vals);
+        for (uint64_t x = base; x < base + q.n; x += rk, ++r) {
+            reinterpret_cast<uint64_t *>(h + o_rb)[r] = x;
+            reinterpret_cast<uint64_t *>(h + o_re)[r] = std::min<uint64_t>(base + q.n, x + rk);
+            reinterpret_cast<uint64_t *>(h + o_rs)[r] = base;
+            reinterpret_cast<uint32_t *>(h + o_rbs)[r] = uint32_t(i);
+            reinterpret_cast<uint32_t *>(h + o_rh)[r] = x == base ? 1u : 0u;
+        }
+        base += q.n;
+    }
+    if (off) {
+        const hipError_t e = hipMemcpyAsync(A.d_tab, h, off, hipMemcpyHostToDevice, streams[s]);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync");
+    }
+    char *d = A.d_tab;
+    GroupDev gd{};
+    gd.sm_k = reinterpret_cast<const uint64_t *>(d + o_smk);
+    gd.sm_v = reinterpret_cast<const uint64_t *>(d + o_smv);
+    gd.sm_len = reinterpret_cast<const uint32_t *>(d + o_sml);
+    gd.md_k = reinterpret_cast<const uint64_t *>(d + o_mdk);
+    gd.md_v = reinterpret_cast<const uint64_t *>(d + o_mdv);
+    gd.r_begin = reinterpret_cast<const uint64_t *>(d + o_rb);
+    gd.r_end = reinterpret_cast<const uint64_t *>(d + o_re);
+    gd.r_sbase = reinterpret_cast<const uint64_t *>(d + o_rs);
+    gd.r_bseg = reinterpret_cast<const uint32_t *>(d + o_rbs);
+    gd.r_head = reinterpret_cast<const uint32_t *>(d + o_rh);
+    gd.rsum = A.rsum;
+    gd.rpre = A.rpre;
+    gd.segtot = A.segtot;
+    const size_t sb = size_t(GROUP_KEYS) * 8;
+    gd.kA = A.stage;
+    gd.kB = A.stage + sb;
+    gd.vA = A.stage + 2 * sb;
+    gd.vB = A.stage + 3 * sb;
+    gd.nsmall = uint32_t(nsm);
+    gd.nmed = uint32_t(nmd);
+    gd.nr = uint32_t(nr);
+    gd.nkeys = base;
+    coalesced += g->reqs.size() > 1 ? g->reqs.size() : 0;
+    return run_group_typed(fr.key_type, vb, gd, fr.hasnan != 0, fr.descending != 0, streams[s]);
+}
 
 bool xss_server::retire()
 {
@@ -228,38 +515,9 @@ bool xss_server::admit()
         g->stream = s;
         const xss_request &fr = *front->r;
         int rc = XSS_OK;
-        if (coalesce_max && fr.op == OP_QSORT && fr.n <= std::min<uint64_t>(coalesce_max, small_limit(fr.key_type))) {
-            // gather queued short qsorts of the same key type and flags
-            uint32_t cnt = 0;
-            for (auto it = queue.begin(); it != queue.end() && cnt < MAX_GROUP;) {
-                const xss_request &q = *(*it)->r;
-                const bool match = q.op == OP_QSORT && q.key_type == fr.key_type && q.hasnan == fr.hasnan &&
-                                   q.descending == fr.descending &&
-                                   q.n <= std::min<uint64_t>(coalesce_max, small_limit(q.key_type));
-                if (match && used + g->footprint + (*it)->footprint <= budget) {
-                    h_ptrs[s][cnt] = reinterpret_cast<uint64_t>(q.keys);
-                    h_len[s][cnt] = uint32_t(q.n);
-                    g->reqs.push_back(*it);
-                    g->footprint += (*it)->footprint;
-                    ++cnt;
-                    it = queue.erase(it);
-                } else {
-                    ++it;
-                }
-            }
-            if (cnt == 0) {  // the head alone exceeds the budget: run it by itself
-                queue.erase(std::find(queue.begin(), queue.end(), front));
-                h_ptrs[s][0] = reinterpret_cast<uint64_t>(fr.keys);
-                h_len[s][0] = uint32_t(fr.n);
-                g->reqs.push_back(front);
-                g->footprint = front->footprint;
-                cnt = 1;
-            }
-            hipError_t e = hipMemcpyAsync(d_ptrs[s], h_ptrs[s], cnt * 8, hipMemcpyHostToDevice, streams[s]);
-            if (e == hipSuccess) e = hipMemcpyAsync(d_len[s], h_len[s], cnt * 4, hipMemcpyHostToDevice, streams[s]);
-            if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync");
-            if (!rc) rc = small_group(fr.key_type, d_ptrs[s], d_len[s], cnt, fr.hasnan, fr.descending, streams[s]);
-            coalesced += cnt > 1 ? cnt : 0;
+        const int vb = group_val_bytes(fr);
+        if (coalesce_max && vb >= 0 && fr.n <= std::min<uint64_t>(coalesce_max, GROUP_KEYS)) {
+            rc = launch_group(s, vb, g);
         } else {
             queue.pop_front();
             g->reqs.push_back(front);
@@ -328,18 +586,17 @@ int xss_server_create(xss_server_t *out, size_t budget_bytes, int nstreams, uint
         XSS_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         sv->streams.push_back(s);
         sv->busy.push_back(false);
-        uint64_t *hp;
-        uint32_t *hl;
-        uint64_t *dp;
-        uint32_t *dl;
-        XSS_HIP_CHECK(hipHostMalloc(&hp, MAX_GROUP * 8));
-        XSS_HIP_CHECK(hipHostMalloc(&hl, MAX_GROUP * 4));
-        XSS_HIP_CHECK(hipMalloc(&dp, MAX_GROUP * 8));
-        XSS_HIP_CHECK(hipMalloc(&dl, MAX_GROUP * 4));
-        sv->h_ptrs.push_back(hp);
-        sv->h_len.push_back(hl);
-        sv->d_ptrs.push_back(dp);
-        sv->d_len.push_back(dl);
+        Arena A;
+        if (coalesce_max) {
+            const size_t tb = Arena::table_bytes();
+            XSS_HIP_CHECK(hipHostMalloc(&A.h_tab, tb));
+            XSS_HIP_CHECK(hipMalloc(&A.d_tab, tb));
+            XSS_HIP_CHECK(hipMalloc(&A.rsum, size_t(GROUP_RANGES) * 256 * 4));
+            XSS_HIP_CHECK(hipMalloc(&A.rpre, size_t(GROUP_RANGES) * 256 * 4));
+            XSS_HIP_CHECK(hipMalloc(&A.segtot, size_t(MAX_GROUP) * 256 * 4));
+            XSS_HIP_CHECK(hipMalloc(&A.stage, size_t(GROUP_KEYS) * 8 * 4));
+        }
+        sv->arenas.push_back(A);
         if (reserve_bytes) {
             // pool: grow this stream's workspace now, not on the first large request
             Workspace *ws = nullptr;
@@ -426,10 +683,10 @@ int xss_server_destroy(xss_server_t sv)
     if (sv->worker.joinable()) sv->worker.join();
     for (size_t i = 0; i < sv->streams.size(); ++i) {
         (void)hipStreamSynchronize(sv->streams[i]);
-        (void)hipHostFree(sv->h_ptrs[i]);
-        (void)hipHostFree(sv->h_len[i]);
-        (void)hipFree(sv->d_ptrs[i]);
-        (void)hipFree(sv->d_len[i]);
+        Arena &A = sv->arenas[i];
+        if (A.h_tab) (void)hipHostFree(A.h_tab);
+        for (void *m : {(void *)A.d_tab, (void *)A.rsum, (void *)A.rpre, (void *)A.segtot, (void *)A.stage})
+            if (m) (void)hipFree(m);
         (void)hipStreamDestroy(sv->streams[i]);
     }
     delete sv;

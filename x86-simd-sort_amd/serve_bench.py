@@ -9,11 +9,14 @@ Request mix (per request, seeded):
   10% qselect (half) / partial_qsort (half) of uint32, length log-uniform in
       [1K, 16M], k uniform in [1, len].
 Arrivals are Poisson at each rate (open loop: a C++ submitter issues every
-request at its arrival time whatever the backlog).  Inputs are device
+request at its arrival time whatever the backlog).  A 200-request trace
+through both modes warms the kernels up first.  Inputs are device
 resident before the trace starts.  The server runs with 4 streams, a 64 GB
-admission budget and coalescing of short qsorts; the baseline serves the
-same trace one request at a time with the plain call.  Prints one JSON line
-per rate: p50/p99 latency and keys/s for both.
+admission budget and coalescing of qsort / keyvalue_qsort requests of at
+most --coalesce keys (default 2^20: LDS-sized ones by the pointer-list sort,
+longer ones by shared segmented passes); the baseline serves the same trace
+one request at a time with the plain call.  Prints one JSON line per rate:
+p50/p99 latency and keys/s for both.
 """
 import argparse
 import json
@@ -112,25 +115,55 @@ def summarize(reqs, wall):
             "mean_ms": round(float(lat.mean()), 3), "keys_per_s": round(keys / wall, 1), "wall_s": round(wall, 4)}
 
 
+MODES = lambda coalesce: [("server", 4, coalesce, 0), ("plain_call_baseline", 1, 0, 1)]  # noqa: E731
+
+
+def warm_up(torch, pools, coalesce):
+    """One short trace through both modes first: HIP loads a kernel's code
+    on its first launch, which would otherwise land in the first requests'
+    latencies."""
+    for name, streams, coal, one in MODES(coalesce):
+        reqs, bufs, _ = make_trace(torch, pools, 200, 50000, seed=1)
+        run_trace(reqs, streams, coal, one)
+        del bufs
+    torch.cuda.empty_cache()
+
+
+def measure(torch, pools, rate, requests, coalesce):
+    out = {"rate_req_s": rate, "requests": requests}
+    for name, streams, coal, one in MODES(coalesce):
+        reqs, bufs, meta = make_trace(torch, pools, requests, rate, seed=rate)
+        wall, stats = run_trace(reqs, streams, coal, one)
+        s = summarize(reqs, wall)
+        s.update({"launch_groups": stats["launch_groups"], "coalesced_requests": stats["coalesced"]})
+        out[name] = s
+        del bufs
+        torch.cuda.empty_cache()
+    out["total_keys"] = sum(int(m[2]) for m in meta)
+    return out
+
+
+def serving_leg(torch, rates=(10000, 50000), requests=1000, coalesce=1 << 20):
+    """The served workload at each rate, server vs one plain call at a time."""
+    pools = Pools(torch)
+    warm_up(torch, pools, coalesce)
+    res = [measure(torch, pools, r, requests, coalesce) for r in rates]
+    del pools
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--rates", default="1000,10000,50000")
     ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--coalesce", type=int, default=1 << 20)
     args = ap.parse_args()
     import torch
     pools = Pools(torch)
+    warm_up(torch, pools, args.coalesce)
     for rate in [int(x) for x in args.rates.split(",")]:
-        out = {"rate_req_s": rate, "requests": args.requests}
-        for name, streams, coal, one in [("server", 4, 4096, 0), ("plain_call_baseline", 1, 0, 1)]:
-            reqs, bufs, meta = make_trace(torch, pools, args.requests, rate, seed=rate)
-            wall, stats = run_trace(reqs, streams, coal, one)
-            s = summarize(reqs, wall)
-            s.update({"launch_groups": stats["launch_groups"], "coalesced_requests": stats["coalesced"]})
-            out[name] = s
-            del bufs
-            torch.cuda.empty_cache()
-        out["total_keys"] = sum(int(m[2]) for m in meta)
-        print(json.dumps(out), flush=True)
+        print(json.dumps(measure(torch, pools, rate, args.requests, args.coalesce)), flush=True)
 
 
 if __name__ == "__main__":

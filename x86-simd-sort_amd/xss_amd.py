@@ -198,7 +198,7 @@ class Server:
     Requests run on the server's own streams: the caller makes their inputs
     ready (e.g. torch.cuda.synchronize()) before submitting them."""
 
-    def __init__(self, budget_bytes=64 << 30, streams=4, coalesce_max=4096, reserve_bytes=0, one_at_a_time=False):
+    def __init__(self, budget_bytes=64 << 30, streams=4, coalesce_max=1 << 20, reserve_bytes=0, one_at_a_time=False):
         L = lib()
         self.h = ctypes.c_void_p()
         _check(L.xss_server_create(ctypes.byref(self.h), _sz(budget_bytes), int(streams),

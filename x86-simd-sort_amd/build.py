@@ -86,6 +86,23 @@ def build(incremental=False, jobs=None):
     return LIB
 
 
+def build_test_libs():
+    """Test libraries built the way a caller builds against include/: the
+    header-only device object_qsort instantiated in tests/cpp/object_gpu.hip
+    (linked to libxss_amd.so, found through its rpath)."""
+    repo = os.path.dirname(HERE)
+    src = os.path.join(repo, "tests", "cpp", "object_gpu.hip")
+    out = os.path.join(repo, "tests", "cpp", "libxss_objtest.so")
+    cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=off",
+           "-I", os.path.join(repo, "include"), src, "-o", out, "-L", HERE, "-lxss_amd",
+           "-Wl,-rpath,$ORIGIN/../../x86-simd-sort_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"test library build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return out
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--incremental", action="store_true")

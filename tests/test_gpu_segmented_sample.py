@@ -1,0 +1,123 @@
+"""GPU parity of the segmented form's sample-sort plan (4-byte keys) against
+the oracle, segment by segment.
+
+Segment lengths sit on both sides of every class boundary of the plan (LDS
+bitonic up to 4096 keys, one-workgroup LDS radix up to 512 x items, sample
+sort above), each segment with its own distribution (constant, two values,
+duplicates, sorted, reverse, NaN, signed zeros), and the plan knobs are
+forced so that every kernel of the path runs: buckets far above the LDS
+capacity (the large-bucket passes), 15-item LDS sorts, wide scatter tiles,
+one sample per bucket."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import xss_testlib as T
+from test_gpu_segmented import Plan, offsets_of
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [0, 1, 2, 3, 100, 4095, 4096, 4097, 6143, 6144, 6145, 8000, 12289, 50000, 100003, 250000, 4099, 9000]
+DISTS = ["random", "dups", "sorted", "reverse", "constant", "two", "smallrange", "nan", "signed_zeros", "rand_max"]
+KNOBS = {
+    "default": {},
+    "big_buckets": {"seg_bucket_keys": 40000},
+    "items15": {"seg_local_items": 15},
+    "wide": {"seg_wide_tiles": 1},
+    "oversample1": {"seg_oversample": 1},
+}
+
+
+def batch(tag, seed):
+    rng = np.random.default_rng(seed)
+    parts = []
+    for i, n in enumerate(SIZES):
+        dist = DISTS[i % len(DISTS)]
+        if dist == "two":
+            pool = T.gen(tag, "random", 2, seed + i)
+            parts.append(pool[rng.integers(0, 2, n)])
+        else:
+            parts.append(T.gen(tag, dist, n, seed + i))
+    keys = np.concatenate(parts).astype(T.TAGS[tag])
+    return keys, offsets_of(np.array(SIZES, dtype=np.int64))
+
+
+@pytest.fixture
+def knobs(xss, request):
+    L = xss.lib()
+    cfg = KNOBS[request.param]
+    for k, v in cfg.items():
+        assert L.xss_gpu_set_tuning(k.encode(), ctypes.c_uint64(v)) == 0
+    yield request.param
+    for k in cfg:
+        L.xss_gpu_set_tuning(k.encode(), ctypes.c_uint64(0))
+
+
+@pytest.mark.parametrize("knobs", list(KNOBS), indirect=True)
+@pytest.mark.parametrize("tag", ["u32", "i32", "f32"])
+@pytest.mark.parametrize("desc", [False, True])
+def test_sample_qsort(xss, knobs, tag, desc):
+    O = T.ORACLE()
+    keys, off = batch(tag, 31)
+    hasnan = tag == "f32"
+    d = xss.DArray.from_numpy(keys)
+    Plan(xss, "qsort", tag, off, range_tiles=2).run(d, hasnan=hasnan, desc=desc)
+    got = d.numpy()
+    for i in range(len(SIZES)):
+        a, b = int(off[i]), int(off[i + 1])
+        T.assert_sorted_equal(got[a:b], O.qsort(keys[a:b], tag, hasnan, desc), tag,
+                              f"{knobs} segment {i} len {b - a} ({DISTS[i % len(DISTS)]})")
+
+
+@pytest.mark.parametrize("knobs", list(KNOBS), indirect=True)
+@pytest.mark.parametrize("tag", ["u32", "i32", "f32"])
+def test_sample_argsort(xss, knobs, tag):
+    """Indices are segment-local, a permutation whose gathered keys equal the
+    oracle's sort; the path is stable, so equal keys keep the input order."""
+    O = T.ORACLE()
+    keys, off = batch(tag, 41)
+    d = xss.DArray.from_numpy(keys)
+    idx = xss.DArray.empty("u32", keys.size)
+    Plan(xss, "argsort", tag, off, range_tiles=2).run(d, idx)
+    got = idx.numpy()
+    assert np.array_equal(T.bits(d.numpy()), T.bits(keys)), "argsort modified the keys"
+    for i in range(len(SIZES)):
+        a, b = int(off[i]), int(off[i + 1])
+        seg = keys[a:b]
+        what = f"{knobs} segment {i} len {b - a} ({DISTS[i % len(DISTS)]})"
+        T.assert_argsort(got[a:b].astype(np.uint64), seg, seg[O.argsort(seg, tag).astype(np.int64)], tag, what)
+        g = T.bits(seg[got[a:b].astype(np.int64)])
+        tie = g[1:] == g[:-1]
+        assert np.all(got[a:b][1:][tie] > got[a:b][:-1][tie]), f"{what}: equal keys out of input order"
+
+
+@pytest.mark.parametrize("knobs", list(KNOBS), indirect=True)
+@pytest.mark.parametrize("tag", ["u32", "f32"])
+def test_sample_qselect(xss, knobs, tag):
+    """Every segment partitioned at k = len // 2 (NaN bits kept)."""
+    O = T.ORACLE()
+    keys, off = batch(tag, 51)
+    hasnan = tag == "f32"
+    d = xss.DArray.from_numpy(keys)
+    Plan(xss, "qselect", tag, off, range_tiles=2).run(d, hasnan=hasnan)
+    got = d.numpy()
+    for i in range(len(SIZES)):
+        a, b = int(off[i]), int(off[i + 1])
+        if b - a < 2:
+            continue
+        k = (b - a) // 2
+        T.assert_selected(got[a:b], keys[a:b], O.qselect(keys[a:b], tag, k, hasnan), k, tag,
+                          f"{knobs} segment {i} len {b - a} ({DISTS[i % len(DISTS)]})")
+
+
+def test_sample_plan_info(xss):
+    """The plan sorts segments above the bitonic limit and up to the LDS radix
+    capacity with one workgroup each (no ranges), the rest by sample sort."""
+    sizes = np.array([10, 5000, 6144, 6145, 100000], dtype=np.int64)
+    off = offsets_of(sizes)
+    p = Plan(xss, "qsort", "f32", off)
+    nsmall, nr, nlarge = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+    assert xss.lib().xss_gpu_segplan_info(p.p, ctypes.byref(nsmall), ctypes.byref(nr), ctypes.byref(nlarge)) == 0
+    assert (nsmall.value, nlarge.value) == (1, 2)
+    assert nr.value >= 2

@@ -498,7 +498,10 @@ struct PassArgs {
 #ifndef XSS_DIG_LDS
 #define XSS_DIG_LDS 0
 #endif
-template <class KD, int PL, int BLOCK, int ITEMS>
+// DIG: keep each sorted slot's digit in an LDS byte array (the default for
+// radix digits is XSS_DIG_LDS; digits that cost more than a shift, such as
+// a sample sort's bucket, are always kept)
+template <class KD, int PL, int BLOCK, int ITEMS, bool DIG = XSS_DIG_LDS != 0>
 struct DownsweepSmem {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
@@ -508,20 +511,30 @@ struct DownsweepSmem {
     uint64_t base[256];          // global position of tile slot 0 of each digit
     uint32_t wsum[WAVES < 4 ? 4 : WAVES];
     U keys[TILE];                // the tile, sorted by digit
-#if XSS_DIG_LDS
-    uint8_t dig[TILE];           // digit of each sorted slot
-#endif
+    uint8_t dig[DIG ? TILE : 1];  // digit of each sorted slot (DIG)
     V vals[PL == PL_NONE ? 1 : TILE];
+};
+
+// The digit of a radix pass: 8 bits of the order key at `shift`.
+template <class KD, bool NL>
+struct RadixDigit {
+    using U = typename KD::U;
+    U flip;
+    int shift;
+    __device__ __forceinline__ uint32_t operator()(U x) const { return KD::digit(x, flip, NL, shift); }
 };
 
 // One tile of a downsweep: load, rank, local sort, write-out.  FULL: all
 // TILE keys valid (no per-item bounds checks); NL: floats with hasnan.
 // The tile holds keys [tbase, tbase + valid) (valid == TILE when FULL).
 // seg_base: start of the segment the tile belongs to (segment-local argsort
-// indices are position - seg_base).
-template <class KD, int PL, int BLOCK, int ITEMS, bool FULL, bool NL, bool UNST = false>
-__device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<KD, PL, BLOCK, ITEMS> &sm,
-                                               uint64_t tbase, uint32_t valid_in, uint64_t seg_base, uint64_t &run)
+// indices are position - seg_base).  dig maps a key to its digit (0..255):
+// the radix digit of the pass, or (the sample sort) its bucket among a
+// segment's splitters.
+template <class KD, int PL, int BLOCK, int ITEMS, bool FULL, bool NL, bool UNST, class DG, bool DIGC>
+__device__ __forceinline__ void downsweep_tile_g(const PassArgs &a, DownsweepSmem<KD, PL, BLOCK, ITEMS, DIGC> &sm,
+                                                 uint64_t tbase, uint32_t valid_in, uint64_t seg_base, uint64_t &run,
+                                                 const DG &dig)
 {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
@@ -530,8 +543,6 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
     constexpr int TILE = BLOCK * ITEMS;
     constexpr int WTILE = 64 * ITEMS;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const U flip = U(a.flip);
-    const int shift = int(a.shift);
     const U *kin = static_cast<const U *>(a.keys_in);
     const V *vin = static_cast<const V *>(a.vals_in);
 
@@ -561,7 +572,7 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
     for (int i = 0; i < ITEMS; ++i) {
         const uint32_t lp = lbase + i * 64;
         const bool real = FULL || lp < valid;
-        const uint32_t d = real ? KD::digit(key[i], flip, NL, shift) : 255u;
+        const uint32_t d = real ? dig(key[i]) : 255u;
         if constexpr (UNST && PL != PL_IDX) {
             // keys only, or key-value (whose payloads inside a run of equal
             // keys may come in any order): each lane reserves its own slot
@@ -639,9 +650,7 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
         const uint32_t d = r >> 16;
         const uint32_t lpos = wc[d] + (r & 0xffffu);
         sm.keys[lpos] = key[i];
-#if XSS_DIG_LDS
-        sm.dig[lpos] = uint8_t(d);
-#endif
+        if constexpr (DIGC) sm.dig[lpos] = uint8_t(d);
         if constexpr (HAS_PL) sm.vals[lpos] = val[i];
     }
     __syncthreads();
@@ -654,12 +663,8 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
         const uint32_t lpos = i * BLOCK + tid;
         if (FULL || lpos < valid) {
             const U x = sm.keys[lpos];
-#if XSS_DIG_LDS
-            const uint64_t g = sm.base[sm.dig[lpos]] + lpos;
-#else
-            // the slot's digit, recomputed from its key (no LDS byte array)
-            const uint64_t g = sm.base[KD::digit(x, flip, NL, shift)] + lpos;
-#endif
+            // the slot's digit: kept in LDS, or recomputed from its key
+            const uint64_t g = sm.base[DIGC ? uint32_t(sm.dig[lpos]) : dig(x)] + lpos;
             if constexpr (PL == PL_IDX) {
                 if (a.last) {
                     if (a.idx32_out) a.idx32_out[g] = uint32_t(sm.vals[lpos] - uint32_t(seg_base));
@@ -675,6 +680,14 @@ __device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<
             if constexpr (HAS_PL) vout[g] = sm.vals[lpos];
         }
     }
+}
+
+template <class KD, int PL, int BLOCK, int ITEMS, bool FULL, bool NL, bool UNST = false, bool DIGC>
+__device__ __forceinline__ void downsweep_tile(const PassArgs &a, DownsweepSmem<KD, PL, BLOCK, ITEMS, DIGC> &sm,
+                                               uint64_t tbase, uint32_t valid_in, uint64_t seg_base, uint64_t &run)
+{
+    const RadixDigit<KD, NL> dig{typename KD::U(a.flip), int(a.shift)};
+    downsweep_tile_g<KD, PL, BLOCK, ITEMS, FULL, NL, UNST>(a, sm, tbase, valid_in, seg_base, run, dig);
 }
 
 // Tile shape and register budget of the scatter passes, per class of item

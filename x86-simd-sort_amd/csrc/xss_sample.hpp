@@ -1,0 +1,755 @@
+// xss_sample.hpp -- segmented sample sort of 4-byte keys: the batch form's
+// segments too long for one workgroup.
+//
+// A radix MSD split of such a segment depends on how its keys spread over
+// the order-key space: uniform floats in [-1, 1], for instance, put half of
+// a segment into two top-digit buckets (their exponents), and those buckets
+// would need further passes.  Splitters taken from a sample of the segment
+// cut it into buckets of about equal size whatever the distribution, so
+// every key is moved once through HBM and then sorted on chip:
+//
+//   ss_sample   per segment: S stratified samples (S = a power of two, about
+//               `oversample` per bucket), bitonic-sorted in LDS by order key;
+//               nb - 1 evenly spaced ones are the splitters
+//   ss_hist     per range (ranges never cross a segment): the bucket of each
+//               key (binary search over the segment's splitters in LDS),
+//               counted into 256 bins -- the digit histogram of a pass
+//   seg_scan    (xss_kernels.hpp) per-segment bucket totals and range bases
+//   ss_scatter  the scatter of a segmented pass with the bucket as the digit
+//               (keys -> scratch; argsort generates its indices here)
+//   ss_buckets  bucket starts and sizes; buckets too large for the LDS sort
+//               are listed (select: only the bucket holding rank len / 2)
+//   ss_local    one workgroup per bucket sorts it by its whole order key in
+//               LDS: the bits where its keys differ (AND / OR of the bucket),
+//               a 12-bit pass whose order inside a digit is free, then
+//               stable 4-bit passes; argsort: stable 4-bit passes only
+//   ss_big      listed buckets: one workgroup runs 8-bit LSD passes over
+//               the bucket in HBM (digits its keys share are skipped)
+//
+// Moved bytes per key: a histogram read, one scatter (read + write) and the
+// LDS sort's read + write: 20 bytes for 4-byte keys, against 36 for four LSD
+// passes.  Everything is planned on the host (segment lengths) or on the
+// device (bucket sizes), with no host wait: a plan's run is capturable.
+#pragma once
+#include "xss_sort.hpp"
+
+namespace xss {
+
+constexpr int SS_SPLIT = 256;         // buckets per segment at most (one scatter digit)
+constexpr int SS_SAMPLE_BLOCK = 1024;
+constexpr uint32_t SS_SAMPLE_MAX = 4096;
+constexpr int SS_LOCAL_BLOCK = 512;
+constexpr int SS_BIG_BLOCK = 1024;
+constexpr int SS_BIG_ITEMS = 8;
+// the scatter's wide tile: 1024 threads x 16 keys (12 with argsort's index)
+constexpr int SS_WIDE_ITEMS_KEYS = 16;
+constexpr int SS_WIDE_ITEMS_IDX = 12;
+template <int PL>
+struct SsWideShape {
+    static constexpr int BLOCK = 1024;
+    static constexpr int ITEMS = PL == PL_IDX ? SS_WIDE_ITEMS_IDX : SS_WIDE_ITEMS_KEYS;
+};
+
+struct SsArgs {
+    // host-built plan
+    const uint64_t *seg_off;  // [nseg + 1]
+    const uint32_t *lseg;     // [nlarge] segment id of each large segment
+    const uint32_t *lnb;      // [nlarge] its buckets (2..256)
+    const uint32_t *lbb;      // [nlarge] its first bucket slot
+    const uint32_t *bk_seg;   // [nbk] large segment of each bucket slot
+    uint32_t oversample;      // samples per bucket
+    // built on the device
+    void *split;              // [nlarge][256] splitters (order keys; 255 used)
+    uint32_t *bstart;         // [nbk] bucket starts (positions in the batch)
+    uint32_t *bsize;          // [nbk]
+    uint32_t *ctl;            // [0]: buckets listed in `big`
+    uint32_t *big;            // [nbk] slots of the buckets larger than lcap
+    uint32_t *ksel;           // [nlarge] select: the slot of the bucket holding rank len / 2
+    uint32_t lcap;            // largest bucket ss_local sorts
+    uint32_t select;          // list only the buckets that hold rank len / 2
+    uint64_t flip;
+    uint32_t nan_last;
+};
+
+// Bucket of order key o among the splitters sp[0..254] (sorted; the entries
+// past the segment's nb - 1 splitters are all-ones): the number of
+// splitters <= o, capped at nb - 1.  Two levels: every 16th splitter
+// (sup[i] = sp[16 i + 15], wave-uniform, in scalar registers) is compared
+// with o in ALU operations, which gives the group of 16; then a 4-step
+// binary search inside the group in LDS (half the LDS lookups of a search
+// over all 255).
+template <class U>
+struct SplitSearch {
+    U sup[15];
+    __device__ __forceinline__ void load(const U *sp)
+    {
+#pragma unroll
+        for (int i = 0; i < 15; ++i) sup[i] = __builtin_amdgcn_readfirstlane(sp[16 * i + 15]);
+    }
+    __device__ __forceinline__ uint32_t operator()(U o, const U *sp, uint32_t nbm1) const
+    {
+        uint32_t g = 0;
+#pragma unroll
+        for (int i = 0; i < 15; ++i) g += sup[i] <= o ? 1u : 0u;
+        uint32_t p = 16 * g;
+#pragma unroll
+        for (uint32_t b = 8; b; b >>= 1)
+            if (sp[p + b - 1] <= o) p += b;
+        return min(p, nbm1);
+    }
+};
+
+template <class KD, bool NL>
+struct SplitDigit {
+    using U = typename KD::U;
+    const U *sp;
+    SplitSearch<U> ss;
+    U flip;
+    uint32_t nbm1;
+    __device__ __forceinline__ uint32_t operator()(U x) const { return ss(KD::ord(x, flip, NL), sp, nbm1); }
+};
+
+__device__ __forceinline__ uint32_t ss_hash(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+// ---------------------------------------------------------------- splitters
+
+// One workgroup per large segment.  Workgroup 0 also clears the list
+// counters of this run (read by kernels later on the stream).
+template <class KD>
+__global__ __launch_bounds__(SS_SAMPLE_BLOCK) void ss_sample_kernel(SsArgs a, const void *keys_v)
+{
+    using U = typename KD::U;
+    __shared__ U s[SS_SAMPLE_MAX];
+    const uint32_t j = blockIdx.x, tid = threadIdx.x;
+    if (j == 0 && tid < 4) a.ctl[tid] = 0;
+    const uint32_t seg = a.lseg[j], nb = a.lnb[j];
+    const uint64_t b0 = a.seg_off[seg], len = a.seg_off[seg + 1] - b0;
+    uint32_t S = 1;
+    while (S < nb * a.oversample && S < SS_SAMPLE_MAX) S <<= 1;
+    while (S > len) S >>= 1;
+    const U *keys = static_cast<const U *>(keys_v) + b0;
+    const U flip = U(a.flip);
+    const bool nl = a.nan_last != 0;
+    // one sample per stratum [i len / S, (i + 1) len / S), at a hashed offset
+    for (uint32_t i = tid; i < S; i += SS_SAMPLE_BLOCK) {
+        const uint64_t lo = uint64_t(i) * len / S, hi = uint64_t(i + 1) * len / S;
+        const uint64_t pos = lo + ss_hash(i * 0x9e3779b9u ^ (j + 1) * 0x85ebca6bu) % (hi - lo);
+        s[i] = KD::ord(keys[pos], flip, nl);
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= S; k <<= 1) {
+        for (uint32_t h = k >> 1; h > 0; h >>= 1) {
+            for (uint32_t i = tid; i < S; i += SS_SAMPLE_BLOCK) {
+                const uint32_t q = i ^ h;
+                if (q > i) {
+                    const U x = s[i], y = s[q];
+                    if ((x > y) == ((i & k) == 0)) {
+                        s[i] = y;
+                        s[q] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    U *out = static_cast<U *>(a.split) + size_t(j) * SS_SPLIT;
+    if (tid < SS_SPLIT - 1) out[tid] = tid + 1 < nb ? s[size_t(tid + 1) * S / nb] : U(KD::ONES);
+}
+
+// ---------------------------------------------------------------- histogram
+
+struct SsRanges {
+    const uint64_t *begin;  // [nr]
+    const uint64_t *end;
+    const uint32_t *lidx;   // [nr] large segment of each range
+};
+
+// One workgroup per range: the bucket counts of its keys (rsum[d][range]).
+template <class KD>
+__global__ __launch_bounds__(256) void ss_hist_kernel(SsArgs a, SsRanges r, const void *keys_v, uint32_t *rsum,
+                                                      uint32_t stride)
+{
+    using U = typename KD::U;
+    __shared__ U sp[SS_SPLIT];
+    __shared__ uint32_t h[256];
+    const uint32_t rg = blockIdx.x, tid = threadIdx.x;
+    const uint32_t j = r.lidx[rg];
+    sp[tid] = tid < SS_SPLIT - 1 ? static_cast<const U *>(a.split)[size_t(j) * SS_SPLIT + tid] : U(KD::ONES);
+    h[tid] = 0;
+    const uint32_t nbm1 = a.lnb[j] - 1;
+    const U flip = U(a.flip);
+    const bool nl = a.nan_last != 0;
+    __syncthreads();
+    SplitSearch<U> search;
+    search.load(sp);
+    const U *keys = static_cast<const U *>(keys_v);
+    const uint64_t r0 = r.begin[rg], r1 = r.end[rg];
+    auto count = [&](U x) {
+        const uint32_t d = search(KD::ord(x, flip, nl), sp, nbm1);
+        const uint64_t act = __ballot(1);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        if (__ballot(d != d0) == 0) {
+            if (lanes_below(act) == 0) atomicAdd(&h[d0], uint32_t(__popcll(act)));
+        } else {
+            atomicAdd(&h[d], 1u);
+        }
+    };
+    constexpr int VEC = 16 / sizeof(U);
+    const uint64_t mis = (reinterpret_cast<uintptr_t>(keys + r0) & 15u) / sizeof(U);
+    const uint64_t head = min(r1 - r0, uint64_t((VEC - mis) % VEC));
+    const uint64_t v0 = r0 + head;
+    const uint64_t nvec = (r1 - v0) / VEC;
+    const uint4 *kv = reinterpret_cast<const uint4 *>(keys + v0);
+    constexpr uint32_t UNR = 2;
+    uint64_t v = tid;
+    for (; v + (UNR - 1) * 256 < nvec; v += UNR * 256) {
+        uint4 q[UNR];
+#pragma unroll
+        for (uint32_t u = 0; u < UNR; ++u) q[u] = kv[v + u * 256];
+#pragma unroll
+        for (uint32_t u = 0; u < UNR; ++u) {
+            const U *e = reinterpret_cast<const U *>(&q[u]);
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) count(e[k]);
+        }
+    }
+    for (; v < nvec; v += 256) {
+        const uint4 q = kv[v];
+        const U *e = reinterpret_cast<const U *>(&q);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) count(e[k]);
+    }
+    const uint64_t t0 = v0 + nvec * VEC;
+    const uint64_t rest = head + (r1 - t0);
+    for (uint64_t i = tid; i < rest; i += 256) count(keys[i < head ? r0 + i : t0 + (i - head)]);
+    __syncthreads();
+    rsum[size_t(tid) * stride + rg] = h[tid];
+}
+
+// ---------------------------------------------------------------- scatter
+
+// One workgroup per range: the tiles of the range scattered to their
+// buckets (positions from seg_scan: segment start + bucket start + the
+// range's base inside the bucket).  UNST: ranked without keeping the order
+// inside a bucket (keys-only sorts: the LDS sort orders every bucket).
+template <class KD, int PL, int BLOCK, int ITEMS, bool NL, bool UNST>
+__global__ __launch_bounds__(BLOCK) void ss_scatter_kernel(PassArgs p, SegRanges r, SsArgs a)
+{
+    using U = typename KD::U;
+    constexpr int TILE = BLOCK * ITEMS;
+    __shared__ DownsweepSmem<KD, PL, BLOCK, ITEMS, true> sm;  // bucket of each slot kept in LDS
+    __shared__ U sp[SS_SPLIT];
+    const uint32_t tid = threadIdx.x, b = blockIdx.x;
+    const uint64_t rb = r.begin[b], re = r.end[b], sbase = r.sbase[b];
+    const uint32_t j = r.rbseg[b];
+    if (tid < SS_SPLIT)
+        sp[tid] = tid < SS_SPLIT - 1 ? static_cast<const U *>(a.split)[size_t(j) * SS_SPLIT + tid] : U(KD::ONES);
+    // bucket starts inside the segment (its barrier also publishes sp)
+    const uint32_t dstart = digit_starts(r.segtot + size_t(j) * 256, sm.wsum);
+    uint64_t run = tid < 256 ? sbase + dstart + r.rpre[size_t(tid) * r.stride + b] : 0u;
+    SplitDigit<KD, NL> dig;
+    dig.sp = sp;
+    dig.ss.load(sp);
+    dig.flip = U(p.flip);
+    dig.nbm1 = a.lnb[j] - 1;
+    uint64_t t = rb;
+    for (; t + TILE <= re; t += TILE)
+        downsweep_tile_g<KD, PL, BLOCK, ITEMS, true, NL, UNST>(p, sm, t, TILE, sbase, run, dig);
+    if (t < re) downsweep_tile_g<KD, PL, BLOCK, ITEMS, false, NL, UNST>(p, sm, t, uint32_t(re - t), sbase, run, dig);
+}
+
+// ---------------------------------------------------------------- bucket table
+
+// One workgroup per large segment, a thread per bucket.
+template <int = 0>
+__global__ __launch_bounds__(256) void ss_buckets_kernel(SsArgs a, const uint32_t *segtot)
+{
+    __shared__ uint32_t wsum[4];
+    const uint32_t j = blockIdx.x, d = threadIdx.x, lane = d & 63, w = d >> 6;
+    const uint32_t nb = a.lnb[j];
+    const uint64_t b0 = a.seg_off[a.lseg[j]], len = a.seg_off[a.lseg[j] + 1] - b0;
+    const uint32_t size = d < nb ? segtot[size_t(j) * 256 + d] : 0u;
+    uint32_t incl = size;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off);
+        if (lane >= uint32_t(off)) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t pre = incl - size;
+    for (uint32_t k = 0; k < w; ++k) pre += wsum[k];
+    if (d >= nb) return;
+    const uint32_t slot = a.lbb[j] + d;
+    a.bstart[slot] = uint32_t(b0) + pre;
+    a.bsize[slot] = size;
+    const uint64_t kth = len / 2;
+    const bool holds_k = pre <= kth && kth < uint64_t(pre) + size;
+    if (a.select && holds_k) a.ksel[j] = slot;
+    if ((!a.select || holds_k) && size > a.lcap) a.big[atomicAdd(a.ctl, 1u)] = slot;
+}
+
+// ---------------------------------------------------------------- LDS sort
+
+struct SsLocalArgs {
+    const uint32_t *slots;    // bucket mode: the slots to sort (null: slot = workgroup)
+    const uint32_t *bstart;
+    const uint32_t *bsize;
+    const uint32_t *bk_seg;
+    const uint32_t *lseg;
+    const uint64_t *seg_off;
+    const uint32_t *med;      // segment mode: segment ids (whole segments, sorted in place)
+    const void *keys_in;      // bucket mode: the scatter's output; segment mode: the keys
+    const void *vals_in;      // bucket mode, argsort: the indices the scatter generated
+    void *keys_out;
+    uint32_t *idx32_out;      // argsort: segment-local indices
+    uint64_t flip;
+    uint32_t canon;           // qsort(hasnan): canonical NaN bits
+};
+
+// Peer mask of the lanes holding the same 4-bit digit (4 ballots; see
+// match_any8).
+__device__ __forceinline__ uint64_t match_any4(uint32_t d)
+{
+    uint32_t lo = ~0u, hi = ~0u;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const uint32_t x = uint32_t(__builtin_amdgcn_sbfe(int32_t(d), b, 1));
+        const uint64_t bal = __builtin_amdgcn_uicmp(x, 0u, 33 /* ne */);
+        lo = __builtin_amdgcn_bitop3_b32(lo, uint32_t(bal), x, 0x90);
+        hi = __builtin_amdgcn_bitop3_b32(hi, uint32_t(bal >> 32), x, 0x90);
+    }
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// One workgroup sorts a bucket (or a whole medium segment) of at most
+// 512 * ITEMS keys by its order key in LDS, and writes it out.  Keys are
+// ranked by r = ord - min (min and max of the bucket): only the bits of
+// max - min are sorted.
+//
+// MSD split, then a rank by comparison: r >> shift, with the shift that
+// makes max - min span 4096 values, splits the keys into 4096 sub-buckets (each key takes its slot with one returning LDS
+// add on 16-bit counters, two to a word; the order inside a sub-bucket is
+// free), which hold about one key each for spread data.  Every key's final
+// place is then its sub-bucket's start plus the number of keys of the
+// sub-bucket that order before it (by order key, ties by slot -- argsort: by
+// original index, so equal keys keep the input order); it is written
+// straight to its place in HBM.
+//
+// A sub-bucket of more than SS_SUB_MAX keys (duplicates, or keys crowded in
+// a small part of the range) would make the comparisons quadratic; the
+// bucket is then sorted LSD by r instead: keys only, the lowest 12 bits in one pass whose
+// order inside a digit is free, then stable 4-bit passes ranked per wave (a
+// 4-bit peer match, per-wave counters); argsort: stable 4-bit passes only.
+constexpr uint32_t SS_SUB_MAX = 32;
+// waves per SIMD the keys-only LDS sort is compiled for (build-time knob)
+#ifndef XSS_SS_LOCAL_WAVES
+#define XSS_SS_LOCAL_WAVES 6
+#endif
+template <class KD, int PL, bool NL, int ITEMS>
+__global__ __launch_bounds__(SS_LOCAL_BLOCK) __attribute__((amdgpu_waves_per_eu(PL == PL_IDX ? 4 : XSS_SS_LOCAL_WAVES)))
+void ss_local_kernel(SsLocalArgs a)
+{
+    using U = typename KD::U;
+    using V = uint32_t;
+    static_assert(sizeof(U) == 4, "4-byte keys");
+    constexpr int BLOCK = SS_LOCAL_BLOCK;
+    constexpr int WAVES = BLOCK / 64;
+    constexpr uint32_t CAP = uint32_t(BLOCK) * ITEMS;
+    constexpr bool IDX = PL == PL_IDX;
+    static_assert(CAP <= 8192, "12-bit ranks fit 13 bits, counts 16 bits");
+    static_assert(WAVES * 16 <= 128, "the fallback's counters are scanned by two waves");
+    __shared__ U s_keys[CAP];
+    __shared__ V s_vals[IDX ? CAP : 1];
+    __shared__ uint32_t s_cnt[2048];  // 4096 16-bit counters / sub-bucket starts
+    __shared__ uint32_t s_wc[WAVES * 16];
+    __shared__ uint32_t s_wsum[WAVES];
+    __shared__ uint32_t s_ao[3];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint64_t st, segb;
+    uint32_t m;
+    if (a.med) {
+        const uint32_t seg = a.med[blockIdx.x];
+        st = a.seg_off[seg];
+        m = uint32_t(a.seg_off[seg + 1] - st);
+        segb = st;
+        if (m > CAP) return;  // the plan sends longer segments elsewhere
+    } else {
+        const uint32_t slot = a.slots ? a.slots[blockIdx.x] : blockIdx.x;
+        st = a.bstart[slot];
+        m = a.bsize[slot];
+        if (m > CAP) return;  // listed for ss_big
+        segb = a.seg_off[a.lseg[a.bk_seg[slot]]];
+    }
+    if (m == 0) return;
+    const U flip = U(a.flip);
+    const U *kin = static_cast<const U *>(a.keys_in) + st;
+    const V *vin = IDX && !a.med ? static_cast<const V *>(a.vals_in) + st : nullptr;
+    U *kout = static_cast<U *>(a.keys_out) + st;
+    uint32_t *iout = a.idx32_out + st;
+    // the key (and index) that ends at bucket position q
+    auto put = [&](uint32_t q, U x, V v) {
+        if constexpr (IDX) {
+            iout[q] = v - uint32_t(segb);
+        } else {
+            if constexpr (KD::KIND == Kind::Float && NL)
+                if (a.canon && KD::is_nan(x)) x = KD::QNAN;
+            kout[q] = x;
+        }
+    };
+    if (tid == 0) {
+        s_ao[0] = ~0u;
+        s_ao[1] = 0u;
+        s_ao[2] = 0u;
+    }
+    // keys into registers, striped: wave w holds positions [w 64 it, (w + 1) 64 it)
+    const uint32_t it = (m + BLOCK - 1) / BLOCK;
+    const uint32_t lbase = wave * 64 * it + lane;
+    U k[ITEMS];
+    V v[ITEMS];
+    U t_min = U(~U(0)), t_max = U(0);
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t idx = lbase + i * 64;
+        const bool real = uint32_t(i) < it && idx < m;
+        k[i] = real ? kin[idx] : U(0);
+        if constexpr (IDX) v[i] = real ? (vin ? vin[idx] : uint32_t(st + idx)) : V(0);
+        if (real) {
+            const U o = KD::ord(k[i], flip, NL);
+            t_min = min(t_min, o);
+            t_max = max(t_max, o);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        t_min = min(t_min, U(__shfl_xor(t_min, off)));
+        t_max = max(t_max, U(__shfl_xor(t_max, off)));
+    }
+    __syncthreads();  // s_ao initialised
+    if (lane == 0) {
+        atomicMin(&s_ao[0], uint32_t(t_min));
+        atomicMax(&s_ao[1], uint32_t(t_max));
+    }
+    __syncthreads();
+    // keys are sorted by r = ord - min, whose range fits `bits` bits
+    const U omin = U(__builtin_amdgcn_readfirstlane(s_ao[0]));
+    const uint32_t range = __builtin_amdgcn_readfirstlane(s_ao[1]) - uint32_t(omin);
+    auto rel = [&](U x) { return uint32_t(KD::ord(x, flip, NL) - omin); };
+    if (range == 0) {  // all keys equal: in input order
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const uint32_t idx = lbase + i * 64;
+            if (uint32_t(i) < it && idx < m) put(idx, k[i], IDX ? v[i] : V(0));
+        }
+        return;
+    }
+    auto start_of = [&](uint32_t d) { return (s_cnt[d >> 1] >> ((d & 1u) * 16u)) & 0xffffu; };
+    // 12-bit split at `sh` of the keys in registers (order inside a digit
+    // free): counts, their scan (largest count to s_ao[2]); the scatter is
+    // separate
+    uint32_t rd[ITEMS];
+    auto rank12 = [&](int sh) {
+        for (uint32_t i = tid; i < 2048; i += BLOCK) s_cnt[i] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            if (uint32_t(i) < it) {
+                const uint32_t idx = lbase + i * 64;
+                const bool real = idx < m;
+                const uint32_t d = (rel(k[i]) >> sh) & 0xfffu;
+                const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+                const bool uni = __ballot(!real || d != d0) == 0;
+                uint32_t old = 0;
+                if (real && (!uni || lane == 0)) {
+                    const uint32_t hs = (d & 1u) * 16u;
+                    old = (atomicAdd(&s_cnt[d >> 1], (uni ? 64u : 1u) << hs) >> hs) & 0xffffu;
+                }
+                rd[i] = old | (d << 13) | (uni ? (1u << 25) : 0u) | (real ? 0u : (1u << 26));
+            }
+        }
+        __syncthreads();
+        // exclusive scan: 8 counters (4 words) per thread
+        uint32_t c[8], sum = 0, mx = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t w = s_cnt[tid * 4 + j];
+            c[2 * j] = w & 0xffffu;
+            c[2 * j + 1] = w >> 16;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            sum += c[j];
+            mx = max(mx, c[j]);
+        }
+        uint32_t incl = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off);
+            if (lane >= uint32_t(off)) incl += t;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx = max(mx, uint32_t(__shfl_xor(mx, off)));
+        if (lane == 63) s_wsum[wave] = incl;
+        if (lane == 0) atomicMax(&s_ao[2], mx);
+        __syncthreads();
+        uint32_t run = incl - sum;
+        for (uint32_t w = 0; w < wave; ++w) run += s_wsum[w];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t lo = run;
+            run += c[2 * j];
+            s_cnt[tid * 4 + j] = lo | (run << 16);
+            run += c[2 * j + 1];
+        }
+        __syncthreads();
+    };
+    auto scatter12 = [&]() {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            if (uint32_t(i) < it) {
+                uint32_t r = rd[i];
+                if (r >> 26) continue;
+                if (r >> 25) r = (__builtin_amdgcn_readfirstlane(r) + lane) & 0x1ffffffu;
+                const uint32_t slot = start_of(r >> 13) + (r & 0x1fffu);
+                s_keys[slot] = k[i];
+                if constexpr (IDX) s_vals[slot] = v[i];
+            }
+        }
+        __syncthreads();
+    };
+    const int bits = 32 - __builtin_clz(range);
+    const int msh = bits > 12 ? bits - 12 : 0;  // the digit (ord - min) >> msh spans [0, 4096)
+    rank12(msh);
+    if (__builtin_amdgcn_readfirstlane(s_ao[2]) <= SS_SUB_MAX) {
+        scatter12();
+        // one item at a time (no register arrays here): keeps the loads of
+        // different items from being hoisted together
+#pragma unroll 1
+        for (uint32_t i = 0; i < it; ++i) {
+            const uint32_t p = lbase + i * 64;
+            if (p < m) {
+                const U x = s_keys[p];
+                const V vx = IDX ? s_vals[p] : V(0);
+                const U ox = KD::ord(x, flip, NL);
+                const uint32_t d = uint32_t(ox - omin) >> msh;
+                const uint32_t lo = start_of(d), hi = d < 4095 ? start_of(d + 1) : m;
+                uint32_t r = 0;
+                for (uint32_t q = lo; q < hi; ++q) {
+                    const U oy = KD::ord(s_keys[q], flip, NL);
+                    bool before = oy < ox;
+                    if (oy == ox) {
+                        if constexpr (IDX) before = s_vals[q] < vx;
+                        else before = q < p;
+                    }
+                    r += before ? 1u : 0u;
+                }
+                put(lo + r, x, vx);
+            }
+        }
+        return;
+    }
+    // LSD fallback.  4 bits at `sh`, stable: items ranked in position order
+    // (wave, item, lane) with a 4-bit peer match and per-wave counters; the
+    // keys come from registers (the first pass) or from LDS.
+    auto pass4 = [&](int sh, bool from_lds) {
+        if (tid < WAVES * 16) s_wc[tid] = 0;
+        __syncthreads();  // also: the previous pass's scatter is complete
+        uint32_t *wc = s_wc + wave * 16;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            if (uint32_t(i) < it) {
+                const uint32_t idx = lbase + i * 64;
+                const bool real = idx < m;
+                if (from_lds && real) {
+                    k[i] = s_keys[idx];
+                    if constexpr (IDX) v[i] = s_vals[idx];
+                }
+                const uint32_t d = (rel(k[i]) >> sh) & 0xfu;
+                const uint64_t peers = match_any4(d) & __ballot(real);
+                uint32_t r = 0;
+                if (real) {
+                    const uint32_t below = lanes_below(peers), tot = uint32_t(__popcll(peers));
+                    const uint32_t c = wc[d];
+                    r = c + below;
+                    if (below + 1 == tot) wc[d] = c + tot;
+                }
+                rd[i] = r | (d << 16);
+            }
+        }
+        __syncthreads();
+        // slot base of every (wave, digit), digit-major then wave: a scan of
+        // the 128 counts held by threads 0..127 (waves 0 and 1)
+        uint32_t c = 0;
+        if (tid < WAVES * 16) c = s_wc[(tid % WAVES) * 16 + tid / WAVES];
+        uint32_t incl = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off);
+            if (lane >= uint32_t(off)) incl += t;
+        }
+        if (tid == 63) s_wsum[0] = incl;
+        __syncthreads();
+        if (tid < WAVES * 16) s_wc[(tid % WAVES) * 16 + tid / WAVES] = (tid >= 64 ? s_wsum[0] : 0u) + incl - c;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            if (uint32_t(i) < it && lbase + i * 64 < m) {
+                const uint32_t slot = wc[rd[i] >> 16] + (rd[i] & 0xffffu);
+                s_keys[slot] = k[i];
+                if constexpr (IDX) s_vals[slot] = v[i];
+            }
+        }
+        __syncthreads();  // the scatter and its reads of s_wc are done
+    };
+    int sh = 0;
+    bool from_lds = false;
+    if constexpr (!IDX) {
+        rank12(sh);
+        scatter12();
+        sh += 12;
+        from_lds = true;
+    }
+    for (; sh < bits; sh += 4) {
+        pass4(sh, from_lds);
+        from_lds = true;
+    }
+    for (uint32_t p = tid; p < m; p += BLOCK) put(p, s_keys[p], IDX ? s_vals[p] : V(0));
+}
+
+// ---------------------------------------------------------------- large buckets
+
+struct SsBigArgs {
+    const uint32_t *ctl;       // [0]: listed buckets
+    const uint32_t *big;
+    const uint32_t *bstart;
+    const uint32_t *bsize;
+    const uint32_t *bk_seg;
+    const uint32_t *lseg;
+    const uint64_t *seg_off;
+    void *y_keys;              // the scatter's output (where each listed bucket is)
+    void *y_vals;              // argsort: its indices
+    void *z_keys;              // the other buffer of the ping-pong: the caller's keys
+    void *z_vals;              // (argsort: a second scratch pair)
+    uint32_t *idx32_out;       // argsort
+    uint64_t flip;
+    uint32_t canon;
+};
+
+// One workgroup per listed bucket: 8-bit LSD passes through HBM over the
+// digits on which the bucket's keys differ, stable, tile by tile as a
+// downsweep does (each pass: a histogram loop, a scan, the tiles).  Keys end
+// in the caller's array (argsort: the last pass writes the indices).
+template <class KD, int PL, bool NL>
+__global__ __launch_bounds__(SS_BIG_BLOCK) void ss_big_kernel(SsBigArgs g)
+{
+    using U = typename KD::U;
+    using V = uint32_t;
+    constexpr int BLOCK = SS_BIG_BLOCK, ITEMS = SS_BIG_ITEMS, TILE = BLOCK * ITEMS;
+    constexpr bool IDX = PL == PL_IDX;
+    __shared__ DownsweepSmem<KD, PL, BLOCK, ITEMS> sm;
+    __shared__ uint32_t h[256];
+    __shared__ uint32_t s_ao[2];
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t nlist = g.ctl[0];
+    const U flip = U(g.flip);
+    for (uint32_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const uint32_t slot = g.big[li];
+        const uint64_t st = g.bstart[slot], m = g.bsize[slot];
+        const uint64_t segb = g.seg_off[g.lseg[g.bk_seg[slot]]];
+        __syncthreads();  // the previous bucket's LDS readers are done
+        if (tid == 0) {
+            s_ao[0] = ~0u;
+            s_ao[1] = 0u;
+        }
+        __syncthreads();
+        const U *yk = static_cast<const U *>(g.y_keys);
+        U t_and = U(~U(0)), t_or = U(0);
+        for (uint64_t p = st + tid; p < st + m; p += BLOCK) {
+            const U o = KD::ord(yk[p], flip, NL);
+            t_and &= o;
+            t_or |= o;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            t_and &= __shfl_xor(t_and, off);
+            t_or |= __shfl_xor(t_or, off);
+        }
+        if (lane == 0) {
+            atomicAnd(&s_ao[0], uint32_t(t_and));
+            atomicOr(&s_ao[1], uint32_t(t_or));
+        }
+        __syncthreads();
+        const uint32_t diff = __builtin_amdgcn_readfirstlane(s_ao[0] ^ s_ao[1]);
+        uint32_t dmask = 0;
+        for (int q = 0; q < KD::PASSES; ++q)
+            if ((diff >> (8 * q)) & 0xffu) dmask |= 1u << q;
+        void *sk = g.y_keys, *sv = g.y_vals, *dk = g.z_keys, *dv = g.z_vals;
+        bool in_y = true;
+        for (int q = 0; q < KD::PASSES; ++q) {
+            if (!((dmask >> q) & 1u)) continue;
+            const bool last = (dmask >> (q + 1)) == 0;
+            // histogram of the digit over the bucket
+            for (uint32_t i = tid; i < 256; i += BLOCK) h[i] = 0;
+            __syncthreads();
+            const U *kin = static_cast<const U *>(sk);
+            for (uint64_t p = st + tid; p < st + m; p += BLOCK) atomicAdd(&h[KD::digit(kin[p], flip, NL, 8 * q)], 1u);
+            __syncthreads();
+            const uint32_t dstart = digit_starts(h, sm.wsum);
+            uint64_t run = tid < 256 ? st + dstart : 0u;
+            PassArgs a{};
+            a.keys_in = sk; a.vals_in = sv; a.keys_out = dk; a.vals_out = dv;
+            a.idx32_out = g.idx32_out; a.shift = uint32_t(8 * q); a.flip = g.flip; a.nan_last = NL;
+            a.first = 0; a.last = last; a.canon_nan = last ? g.canon : 0u;
+            uint64_t t = st;
+            for (; t + TILE <= st + m; t += TILE) downsweep_tile<KD, PL, BLOCK, ITEMS, true, NL>(a, sm, t, TILE, segb, run);
+            if (t < st + m) downsweep_tile<KD, PL, BLOCK, ITEMS, false, NL>(a, sm, t, uint32_t(st + m - t), segb, run);
+            __syncthreads();
+            void *tk = sk, *tv = sv;
+            sk = dk;
+            sv = dv;
+            dk = tk;
+            dv = tv;
+            in_y = !in_y;
+        }
+        // keys left in the scatter's buffer (an even number of passes, or
+        // none): copied to the caller's array (argsort: the indices written)
+        if constexpr (IDX) {
+            if (dmask == 0) {
+                const V *yv = static_cast<const V *>(g.y_vals);
+                for (uint64_t p = st + tid; p < st + m; p += BLOCK) g.idx32_out[p] = yv[p] - uint32_t(segb);
+            }
+        } else {
+            if (in_y) {
+                const U *src = static_cast<const U *>(g.y_keys);
+                U *dst = static_cast<U *>(g.z_keys);
+                for (uint64_t p = st + tid; p < st + m; p += BLOCK) {
+                    U x = src[p];
+                    if constexpr (KD::KIND == Kind::Float && NL)
+                        if (g.canon && KD::is_nan(x)) x = KD::QNAN;
+                    dst[p] = x;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- select
+
+// Copy of [begin, end) of every range (the select's scatter output back to
+// the caller's keys; the bucket holding rank len / 2 is then sorted over it).
+template <class U>
+__global__ __launch_bounds__(256) void ss_copy_kernel(SsRanges r, const U *src, U *dst)
+{
+    const uint64_t r0 = r.begin[blockIdx.x], r1 = r.end[blockIdx.x];
+    for (uint64_t p = r0 + threadIdx.x; p < r1; p += 256) dst[p] = src[p];
+}
+
+}  // namespace xss

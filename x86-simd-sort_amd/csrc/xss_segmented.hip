@@ -6,19 +6,27 @@
 // segments by size once:
 //   * small segments (len <= small_max): one LDS bitonic workgroup each
 //     (block_sort_kernel over a segment list) -- one launch for all of them;
-//   * larger segments: cut into ranges of range_tiles downsweep tiles; every
-//     LSD pass is upsweep (one workgroup per range) -> seg_scan_kernel (per
-//     digit, an exclusive scan over the ranges of each segment, restarting at
-//     every segment) -> seg_downsweep_kernel (one workgroup per range, running
-//     per-digit positions inside its segment).  A 1M-key segment is spread
-//     over many workgroups; 4K-key segments take one range each.
+//   * 4-byte keys (u32 / i32 / f32):
+//       - medium segments (up to the LDS radix sort's capacity): one
+//         workgroup each sorts the segment by its order key in LDS
+//         (ss_local_kernel, xss_sample.hpp);
+//       - large segments: a sample sort (xss_sample.hpp) -- splitters from a
+//         sorted sample, one histogram read and one scatter into buckets of
+//         about `bucket_keys` keys, then every bucket sorted in LDS; qselect
+//         sorts only the bucket holding rank len / 2;
+//   * other key widths, larger segments: cut into ranges of range_tiles
+//     downsweep tiles; every LSD pass is upsweep (one workgroup per range) ->
+//     seg_scan_kernel (per digit, an exclusive scan over the ranges of each
+//     segment, restarting at every segment) -> seg_downsweep_kernel (one
+//     workgroup per range, running per-digit positions inside its segment).
 // The plan owns all device memory it needs, so xss_gpu_segplan_run allocates
 // nothing and its launches can be captured in a hipGraph and replayed.
+#include <atomic>
 #include <memory>
 #include <string>
 #include <vector>
 
-#include "xss_sort.hpp"
+#include "xss_sample.hpp"
 
 using namespace xss;
 
@@ -27,9 +35,38 @@ namespace {
 struct Tuning {
     uint32_t small_max;    // segments up to this length: LDS bitonic
     uint32_t range_tiles;  // tiles per workgroup range in larger segments
+    uint32_t bucket_keys;  // sample sort: keys per bucket (average)
+    uint32_t oversample;   // sample sort: samples per bucket
+    uint32_t local_items;  // LDS sort: keys per thread, 12 or 15 (capacity 512 x items)
+    uint32_t wide_tiles;   // sample sort's scatter: 1024-thread tiles (else 512)
 };
 
 }  // namespace
+
+namespace xss {
+
+// Plan knobs of the segmented form's sample sort (xss_gpu_set_tuning,
+// read when a plan is created); 0 = the default.
+std::atomic<uint64_t> &seg_knob(int which)
+{
+    static std::atomic<uint64_t> v[5];
+    return v[which];
+}
+enum { SEG_BUCKET_KEYS = 0, SEG_OVERSAMPLE, SEG_LOCAL_ITEMS, SEG_WIDE_TILES, SEG_SAMPLE_OFF };
+
+bool seg_tuning_set(const std::string &name, uint64_t value)
+{
+    static const char *const names[5] = {"seg_bucket_keys", "seg_oversample", "seg_local_items", "seg_wide_tiles",
+                                         "seg_sample_off"};
+    for (int i = 0; i < 5; ++i)
+        if (name == names[i]) {
+            seg_knob(i).store(value);
+            return true;
+        }
+    return false;
+}
+
+}  // namespace xss
 
 // ------------------------------------------------------------------ plan
 
@@ -47,6 +84,12 @@ struct xss_segplan {
     uint32_t *r_bseg = nullptr, *r_head = nullptr;
     uint32_t *rsum = nullptr, *rpre = nullptr, *segtot = nullptr;
     char *scratch = nullptr;    // ping-pong keys (+ indices)
+    // sample sort (4-byte keys)
+    bool sample = false;
+    uint32_t nmed = 0, nbk = 0, lcap = 0;
+    uint32_t *med_list = nullptr, *lseg = nullptr, *lnb = nullptr, *lbb = nullptr, *bk_seg = nullptr;
+    void *split = nullptr;
+    uint32_t *bstart = nullptr, *bsize = nullptr, *ctl = nullptr, *big = nullptr, *ksel = nullptr;
 };
 
 namespace {
@@ -130,9 +173,139 @@ int seg_run(xss_segplan *p, void *keys, uint32_t *idx32, bool hasnan, bool desc,
     return XSS_OK;
 }
 
+// sample-sort plan of 4-byte keys: small segments (bitonic), medium (LDS
+// radix, in place), large (sample sort: keys -> scratch -> keys)
+template <class KD, int PL, bool NL, int ITEMS>
+int ss_run(xss_segplan *p, void *keys, uint32_t *idx32, bool desc, hipStream_t s)
+{
+    using U = typename KD::U;
+    using BS = BlockShape<KD, PL>;
+    constexpr bool IDX = PL == PL_IDX;
+    const uint64_t flip = desc ? ~0ull : 0ull;
+    const uint32_t canon = (p->op == 0 && NL) ? 1u : 0u;
+    if (p->nsmall) {
+        BlockSortArgs b{};
+        b.keys_in = keys; b.keys_out = keys; b.idx32_out = idx32; b.seg_off = p->seg_off;
+        b.seg_list = p->small_list; b.nan_last = NL; b.flip = flip; b.canon_nan = canon;
+        KernelTimer t("seg_small", s);
+        hipLaunchKernelGGL((block_sort_kernel<KD, PL, BS::BLOCK, BS::MAXN>), dim3(p->nsmall), dim3(BS::BLOCK), 0, s, b);
+        XSS_HIP_CHECK(hipGetLastError());
+    }
+    // scratch: the scatter's output (keys, argsort indices) and, for
+    // argsort, the other buffer pair of the large-bucket passes
+    char *yk = p->scratch, *yv = nullptr, *zk = static_cast<char *>(keys), *zv = nullptr;
+    if (IDX) {
+        yv = yk + align256(p->n * sizeof(U));
+        zk = yv + align256(p->n * 4);
+        zv = zk + align256(p->n * sizeof(U));
+    }
+    SsLocalArgs la{};
+    la.bstart = p->bstart; la.bsize = p->bsize; la.bk_seg = p->bk_seg; la.lseg = p->lseg; la.seg_off = p->seg_off;
+    la.idx32_out = idx32; la.flip = flip; la.canon = canon; la.keys_out = keys;
+    if (p->nmed) {
+        SsLocalArgs m = la;
+        m.med = p->med_list; m.keys_in = keys;
+        KernelTimer t("seg_medium", s);
+        hipLaunchKernelGGL((ss_local_kernel<KD, PL, NL, ITEMS>), dim3(p->nmed), dim3(SS_LOCAL_BLOCK), 0, s, m);
+        XSS_HIP_CHECK(hipGetLastError());
+    }
+    if (!p->nbig) return XSS_OK;
+    SsArgs a{};
+    a.seg_off = p->seg_off; a.lseg = p->lseg; a.lnb = p->lnb; a.lbb = p->lbb; a.bk_seg = p->bk_seg;
+    a.oversample = p->tune.oversample; a.split = p->split; a.bstart = p->bstart; a.bsize = p->bsize;
+    a.ctl = p->ctl; a.big = p->big; a.ksel = p->ksel; a.lcap = p->lcap; a.select = p->op == 2;
+    a.flip = flip; a.nan_last = NL;
+    {
+        KernelTimer t("ss_sample", s);
+        hipLaunchKernelGGL((ss_sample_kernel<KD>), dim3(p->nbig), dim3(SS_SAMPLE_BLOCK), 0, s, a,
+                           static_cast<const void *>(keys));
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    {
+        KernelTimer t("ss_hist", s);
+        hipLaunchKernelGGL((ss_hist_kernel<KD>), dim3(p->nr), dim3(256), 0, s, a, SsRanges{p->r_begin, p->r_end, p->r_bseg},
+                           static_cast<const void *>(keys), p->rsum, p->nr);
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    SegScanArgs sc{p->rsum, p->rpre, p->segtot, p->r_head, p->r_bseg, p->nr, p->nr, nullptr};
+    {
+        KernelTimer t("seg_scan", s);
+        hipLaunchKernelGGL(seg_scan_kernel<0>, dim3(256), dim3(256), 0, s, sc);
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    {
+        PassArgs pa{};
+        pa.keys_in = keys; pa.keys_out = yk; pa.vals_out = yv; pa.n = uint32_t(p->n); pa.flip = flip;
+        pa.nan_last = NL; pa.first = 1; pa.last = 0;
+        SegRanges r{p->r_begin, p->r_end, p->r_sbase, p->r_bseg, p->rpre, p->segtot, p->nr, p->nr, nullptr};
+        constexpr bool UN = !IDX;  // keys only: the LDS sort orders each bucket
+        KernelTimer t("ss_scatter", s);
+        if (p->tune.wide_tiles) {
+            using SS = SsWideShape<PL>;
+            hipLaunchKernelGGL((ss_scatter_kernel<KD, PL, SS::BLOCK, SS::ITEMS, NL, UN>), dim3(p->nr), dim3(SS::BLOCK), 0,
+                               s, pa, r, a);
+        } else {
+            using SS = SegShape<KD, PL>;
+            hipLaunchKernelGGL((ss_scatter_kernel<KD, PL, SS::BLOCK, SS::ITEMS, NL, UN>), dim3(p->nr), dim3(SS::BLOCK), 0,
+                               s, pa, r, a);
+        }
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    {
+        KernelTimer t("ss_buckets", s);
+        hipLaunchKernelGGL(ss_buckets_kernel<0>, dim3(p->nbig), dim3(256), 0, s, a, static_cast<const uint32_t *>(p->segtot));
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    la.keys_in = yk; la.vals_in = yv;
+    if (p->op == 2) {
+        // select: the partition back in the caller's keys, then the bucket
+        // holding rank len / 2 of every large segment sorted over it
+        KernelTimer t("ss_copy", s);
+        hipLaunchKernelGGL(ss_copy_kernel<U>, dim3(p->nr), dim3(256), 0, s, SsRanges{p->r_begin, p->r_end, p->r_bseg},
+                           reinterpret_cast<const U *>(yk), static_cast<U *>(keys));
+        XSS_HIP_CHECK(hipGetLastError());
+        la.slots = p->ksel;
+    }
+    {
+        KernelTimer t("ss_local", s);
+        hipLaunchKernelGGL((ss_local_kernel<KD, PL, NL, ITEMS>), dim3(p->op == 2 ? p->nbig : p->nbk),
+                           dim3(SS_LOCAL_BLOCK), 0, s, la);
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    {
+        SsBigArgs g{};
+        g.ctl = p->ctl; g.big = p->big; g.bstart = p->bstart; g.bsize = p->bsize; g.bk_seg = p->bk_seg; g.lseg = p->lseg;
+        g.seg_off = p->seg_off; g.y_keys = yk; g.y_vals = yv; g.z_keys = zk; g.z_vals = zv; g.idx32_out = idx32;
+        g.flip = flip; g.canon = canon;
+        KernelTimer t("ss_big", s);
+        hipLaunchKernelGGL((ss_big_kernel<KD, PL, NL>), dim3(std::min<uint32_t>(p->nbk, 256)), dim3(SS_BIG_BLOCK), 0, s,
+                           g);
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    return XSS_OK;
+}
+
+template <class KD, int PL>
+int ss_dispatch(xss_segplan *p, void *keys, uint32_t *idx32, bool hasnan, bool desc, hipStream_t s)
+{
+    const bool nl = (KD::KIND == Kind::Float) && hasnan;
+    if (p->tune.local_items == 15) {
+        if (nl) return ss_run<KD, PL, true, 15>(p, keys, idx32, desc, s);
+        return ss_run<KD, PL, false, 15>(p, keys, idx32, desc, s);
+    }
+    if (nl) return ss_run<KD, PL, true, 12>(p, keys, idx32, desc, s);
+    return ss_run<KD, PL, false, 12>(p, keys, idx32, desc, s);
+}
+
 template <class KD>
 int seg_dispatch_op(xss_segplan *p, void *keys, uint32_t *idx32, bool hasnan, bool desc, hipStream_t s)
 {
+    if constexpr (sizeof(typename KD::U) == 4) {
+        if (p->sample) {
+            if (p->op == 1) return ss_dispatch<KD, PL_IDX>(p, keys, idx32, hasnan, desc, s);
+            return ss_dispatch<KD, PL_NONE>(p, keys, nullptr, hasnan, desc, s);
+        }
+    }
     if (p->op == 1) return seg_run<KD, PL_IDX>(p, keys, idx32, hasnan, desc, s);
     return seg_run<KD, PL_NONE>(p, keys, nullptr, hasnan, desc, s);
 }
@@ -189,9 +362,27 @@ int xss_gpu_segplan_create(xss_segplan_t *out, const char *op, const char *key_t
     shape_of(p->tag, p->op, &maxn, &tile);
     p->tune.small_max = small_max ? std::min(small_max, maxn) : maxn;
     p->tune.range_tiles = range_tiles ? range_tiles : 8;
+    // sample sort of 4-byte keys (unless switched off): plan knobs
+    p->sample = kb == 4 && seg_knob(SEG_SAMPLE_OFF).load() == 0;
+    if (p->sample) {
+        const uint64_t items = seg_knob(SEG_LOCAL_ITEMS).load();
+        p->tune.local_items = items == 15 ? 15 : 12;
+        p->lcap = uint32_t(SS_LOCAL_BLOCK) * p->tune.local_items;
+        const uint64_t bk = seg_knob(SEG_BUCKET_KEYS).load();
+        // measured on 8192 arrays of log-uniform length: 4096-key buckets,
+        // 32 samples per bucket (capped at SS_SAMPLE_MAX per segment)
+        p->tune.bucket_keys = bk ? uint32_t(std::min<uint64_t>(bk, 1u << 30)) : p->lcap * 2 / 3;
+        const uint64_t ov = seg_knob(SEG_OVERSAMPLE).load();
+        p->tune.oversample = ov ? uint32_t(std::min<uint64_t>(ov, SS_SAMPLE_MAX)) : 32;
+        p->tune.wide_tiles = seg_knob(SEG_WIDE_TILES).load() ? 1 : 0;
+        // the scatter's tile (SegShape, or HybSegShape with wide tiles)
+        const bool idx = p->op == 1;
+        tile = p->tune.wide_tiles ? 1024u * (idx ? SS_WIDE_ITEMS_IDX : SS_WIDE_ITEMS_KEYS)
+                                  : (idx ? 512u * 16u : uint32_t(XSS_SEG_BLOCK_C1) * XSS_SEG_ITEMS_C1);
+    }
     const uint64_t rkeys = uint64_t(p->tune.range_tiles) * tile;
     // bucket the segments
-    std::vector<uint32_t> small;
+    std::vector<uint32_t> small, med, lseg, lnb, lbb, bkseg;
     std::vector<uint64_t> rb, re, rs;
     std::vector<uint32_t> rbs, rh;
     uint32_t nbig = 0;
@@ -205,6 +396,18 @@ int xss_gpu_segplan_create(xss_segplan_t *out, const char *op, const char *key_t
             small.push_back(uint32_t(i));
             continue;
         }
+        if (p->sample && len <= p->lcap) {
+            med.push_back(uint32_t(i));
+            continue;
+        }
+        if (p->sample) {
+            const uint64_t nb = std::min<uint64_t>(SS_SPLIT, std::max<uint64_t>(2, (len + p->tune.bucket_keys - 1) /
+                                                                                     p->tune.bucket_keys));
+            lseg.push_back(uint32_t(i));
+            lnb.push_back(uint32_t(nb));
+            lbb.push_back(uint32_t(bkseg.size()));
+            bkseg.insert(bkseg.end(), nb, nbig);
+        }
         for (uint64_t x = b; x < b + len; x += rkeys) {
             rb.push_back(x);
             re.push_back(std::min(b + len, x + rkeys));
@@ -215,10 +418,12 @@ int xss_gpu_segplan_create(xss_segplan_t *out, const char *op, const char *key_t
         ++nbig;
     }
     p->nsmall = uint32_t(small.size());
+    p->nmed = uint32_t(med.size());
     p->nr = uint32_t(rb.size());
     p->nbig = nbig;
-    // device memory: metadata + ping-pong scratch
-    const size_t nr = p->nr;
+    p->nbk = uint32_t(bkseg.size());
+    // device memory: metadata + scratch
+    const size_t nr = p->nr, nbk = p->nbk;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t o2 = off;
@@ -229,8 +434,17 @@ int xss_gpu_segplan_create(xss_segplan_t *out, const char *op, const char *key_t
     const size_t o_rb = take(nr * 8 + 8), o_re = take(nr * 8 + 8), o_rs = take(nr * 8 + 8);
     const size_t o_rbs = take(nr * 4 + 4), o_rh = take(nr * 4 + 4);
     const size_t o_rsum = take(nr * 256 * 4 + 4), o_rpre = take(nr * 256 * 4 + 4), o_tot = take(size_t(nbig) * 1024 + 4);
-    const size_t scratch = p->nr ? (p->op == 1 ? 2 * (align256(p->n * kb) + align256(p->n * 4)) : align256(p->n * kb)) : 0;
+    size_t scratch = 0;
+    if (p->nr) {
+        // LSD: keys ping-pong (argsort: two key + index pairs); sample sort:
+        // the scatter's output (argsort: two key + index pairs as well)
+        scratch = p->op == 1 ? 2 * (align256(p->n * kb) + align256(p->n * 4)) : align256(p->n * kb);
+    }
     const size_t o_scr = take(scratch + 256);
+    const size_t o_med = take(med.size() * 4 + 4), o_lseg = take(size_t(nbig) * 4 + 4), o_lnb = take(size_t(nbig) * 4 + 4);
+    const size_t o_lbb = take(size_t(nbig) * 4 + 4), o_bks = take(nbk * 4 + 4), o_split = take(size_t(nbig) * SS_SPLIT * 4 + 4);
+    const size_t o_bst = take(nbk * 4 + 4), o_bsz = take(nbk * 4 + 4), o_big = take(nbk * 4 + 4);
+    const size_t o_ksel = take(size_t(nbig) * 4 + 4), o_ctl = take(16);
     XSS_HIP_CHECK(hipGetDevice(&p->device));
     XSS_HIP_CHECK(hipMalloc(&p->mem, off));
     char *m = p->mem;
@@ -245,15 +459,34 @@ int xss_gpu_segplan_create(xss_segplan_t *out, const char *op, const char *key_t
     p->rpre = reinterpret_cast<uint32_t *>(m + o_rpre);
     p->segtot = reinterpret_cast<uint32_t *>(m + o_tot);
     p->scratch = m + o_scr;
-    if (nseg) XSS_HIP_CHECK(hipMemcpy(p->seg_off, offsets, (nseg + 1) * 8, hipMemcpyHostToDevice));
-    if (!small.empty()) XSS_HIP_CHECK(hipMemcpy(p->small_list, small.data(), small.size() * 4, hipMemcpyHostToDevice));
-    if (nr) {
-        XSS_HIP_CHECK(hipMemcpy(p->r_begin, rb.data(), nr * 8, hipMemcpyHostToDevice));
-        XSS_HIP_CHECK(hipMemcpy(p->r_end, re.data(), nr * 8, hipMemcpyHostToDevice));
-        XSS_HIP_CHECK(hipMemcpy(p->r_sbase, rs.data(), nr * 8, hipMemcpyHostToDevice));
-        XSS_HIP_CHECK(hipMemcpy(p->r_bseg, rbs.data(), nr * 4, hipMemcpyHostToDevice));
-        XSS_HIP_CHECK(hipMemcpy(p->r_head, rh.data(), nr * 4, hipMemcpyHostToDevice));
-    }
+    p->med_list = reinterpret_cast<uint32_t *>(m + o_med);
+    p->lseg = reinterpret_cast<uint32_t *>(m + o_lseg);
+    p->lnb = reinterpret_cast<uint32_t *>(m + o_lnb);
+    p->lbb = reinterpret_cast<uint32_t *>(m + o_lbb);
+    p->bk_seg = reinterpret_cast<uint32_t *>(m + o_bks);
+    p->split = m + o_split;
+    p->bstart = reinterpret_cast<uint32_t *>(m + o_bst);
+    p->bsize = reinterpret_cast<uint32_t *>(m + o_bsz);
+    p->big = reinterpret_cast<uint32_t *>(m + o_big);
+    p->ksel = reinterpret_cast<uint32_t *>(m + o_ksel);
+    p->ctl = reinterpret_cast<uint32_t *>(m + o_ctl);
+    auto up = [](void *dst, const void *src, size_t bytes) {
+        return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    XSS_HIP_CHECK(up(p->seg_off, offsets, nseg ? (nseg + 1) * 8 : 0));
+    XSS_HIP_CHECK(up(p->small_list, small.data(), small.size() * 4));
+    XSS_HIP_CHECK(up(p->r_begin, rb.data(), nr * 8));
+    XSS_HIP_CHECK(up(p->r_end, re.data(), nr * 8));
+    XSS_HIP_CHECK(up(p->r_sbase, rs.data(), nr * 8));
+    XSS_HIP_CHECK(up(p->r_bseg, rbs.data(), nr * 4));
+    XSS_HIP_CHECK(up(p->r_head, rh.data(), nr * 4));
+    XSS_HIP_CHECK(up(p->med_list, med.data(), med.size() * 4));
+    XSS_HIP_CHECK(up(p->lseg, lseg.data(), lseg.size() * 4));
+    XSS_HIP_CHECK(up(p->lnb, lnb.data(), lnb.size() * 4));
+    XSS_HIP_CHECK(up(p->lbb, lbb.data(), lbb.size() * 4));
+    XSS_HIP_CHECK(up(p->bk_seg, bkseg.data(), bkseg.size() * 4));
+    // select's slot per segment starts valid (slot 0) before any run writes it
+    XSS_HIP_CHECK(hipMemset(p->ksel, 0, size_t(nbig) * 4 + 4));
     *out = p.release();
     return XSS_OK;
 }

@@ -41,11 +41,14 @@ std::atomic<uint64_t> &count_min_n()
     static std::atomic<uint64_t> v{CS_MIN_N};
     return v;
 }
+// plan knobs of the segmented form (xss_segmented.hip)
+bool seg_tuning_set(const std::string &name, uint64_t value);
 }  // namespace xss
 
 extern "C" int xss_gpu_set_tuning(const char *name, uint64_t value)
 {
     if (!name) return set_error(XSS_ERR_INVALID, "null tuning name");
+    if (seg_tuning_set(name, value)) return XSS_OK;
     if (std::string(name) == "hybrid_min_n") {
         hybrid_min_n().store(value);
         return XSS_OK;

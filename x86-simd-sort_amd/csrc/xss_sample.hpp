@@ -62,6 +62,8 @@ struct SsArgs {
     void *split;              // [nlarge][256] splitters (order keys; 255 used)
     uint32_t *bstart;         // [nbk] bucket starts (positions in the batch)
     uint32_t *bsize;          // [nbk]
+    uint32_t *brange;         // [nbk][2] lowest order key and key range of the
+                              // bucket from its splitters ([1] = 0: unknown)
     uint32_t *ctl;            // [0]: buckets listed in `big`
     uint32_t *big;            // [nbk] slots of the buckets larger than lcap
     uint32_t *ksel;           // [nlarge] select: the slot of the bucket holding rank len / 2
@@ -291,6 +293,12 @@ __global__ __launch_bounds__(256) void ss_buckets_kernel(SsArgs a, const uint32_
     const uint32_t slot = a.lbb[j] + d;
     a.bstart[slot] = uint32_t(b0) + pre;
     a.bsize[slot] = size;
+    // a middle bucket holds order keys in [sp[d - 1], sp[d]); the first and
+    // last are open-ended (their LDS sort measures min and max)
+    const uint32_t *sp = static_cast<const uint32_t *>(a.split) + size_t(j) * SS_SPLIT;
+    const bool mid = d > 0 && d + 1 < nb && sp[d] > sp[d - 1];
+    a.brange[2 * size_t(slot)] = mid ? sp[d - 1] : 0u;
+    a.brange[2 * size_t(slot) + 1] = mid ? sp[d] - 1u - sp[d - 1] : 0u;
     const uint64_t kth = len / 2;
     const bool holds_k = pre <= kth && kth < uint64_t(pre) + size;
     if (a.select && holds_k) a.ksel[j] = slot;
@@ -303,6 +311,7 @@ struct SsLocalArgs {
     const uint32_t *slots;    // bucket mode: the slots to sort (null: slot = workgroup)
     const uint32_t *bstart;
     const uint32_t *bsize;
+    const uint32_t *brange;   // [nbk][2] key range from the splitters ([1] = 0: unknown)
     const uint32_t *bk_seg;
     const uint32_t *lseg;
     const uint64_t *seg_off;
@@ -375,7 +384,7 @@ void ss_local_kernel(SsLocalArgs a)
     __shared__ uint32_t s_ao[3];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint64_t st, segb;
-    uint32_t m;
+    uint32_t m, known_lo = 0, known_range = 0;
     if (a.med) {
         const uint32_t seg = a.med[blockIdx.x];
         st = a.seg_off[seg];
@@ -386,8 +395,10 @@ void ss_local_kernel(SsLocalArgs a)
         const uint32_t slot = a.slots ? a.slots[blockIdx.x] : blockIdx.x;
         st = a.bstart[slot];
         m = a.bsize[slot];
+        known_lo = a.brange[2 * size_t(slot)];
+        known_range = a.brange[2 * size_t(slot) + 1];
         if (m > CAP) return;  // listed for ss_big
-        segb = a.seg_off[a.lseg[a.bk_seg[slot]]];
+        segb = IDX ? a.seg_off[a.lseg[a.bk_seg[slot]]] : 0;
     }
     if (m == 0) return;
     const U flip = U(a.flip);
@@ -405,6 +416,8 @@ void ss_local_kernel(SsLocalArgs a)
             kout[q] = x;
         }
     };
+    // the bucket's key range: from its splitters, or measured (min / max)
+    const bool measure = known_range == 0;
     if (tid == 0) {
         s_ao[0] = ~0u;
         s_ao[1] = 0u;
@@ -422,26 +435,30 @@ void ss_local_kernel(SsLocalArgs a)
         const bool real = uint32_t(i) < it && idx < m;
         k[i] = real ? kin[idx] : U(0);
         if constexpr (IDX) v[i] = real ? (vin ? vin[idx] : uint32_t(st + idx)) : V(0);
-        if (real) {
+        if (measure && real) {
             const U o = KD::ord(k[i], flip, NL);
             t_min = min(t_min, o);
             t_max = max(t_max, o);
         }
     }
+    if (measure) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        t_min = min(t_min, U(__shfl_xor(t_min, off)));
-        t_max = max(t_max, U(__shfl_xor(t_max, off)));
+        for (int off = 32; off > 0; off >>= 1) {
+            t_min = min(t_min, U(__shfl_xor(t_min, off)));
+            t_max = max(t_max, U(__shfl_xor(t_max, off)));
+        }
+        __syncthreads();  // s_ao initialised
+        if (lane == 0) {
+            atomicMin(&s_ao[0], uint32_t(t_min));
+            atomicMax(&s_ao[1], uint32_t(t_max));
+        }
+        __syncthreads();
     }
-    __syncthreads();  // s_ao initialised
-    if (lane == 0) {
-        atomicMin(&s_ao[0], uint32_t(t_min));
-        atomicMax(&s_ao[1], uint32_t(t_max));
-    }
-    __syncthreads();
-    // keys are sorted by r = ord - min, whose range fits `bits` bits
-    const U omin = U(__builtin_amdgcn_readfirstlane(s_ao[0]));
-    const uint32_t range = __builtin_amdgcn_readfirstlane(s_ao[1]) - uint32_t(omin);
+    // keys are sorted by r = ord - min, whose range fits `bits` bits (a
+    // range from the splitters may include values no key has: then the
+    // bucket is sorted as if keys were spread over all of it)
+    const U omin = measure ? U(__builtin_amdgcn_readfirstlane(s_ao[0])) : U(known_lo);
+    const uint32_t range = measure ? __builtin_amdgcn_readfirstlane(s_ao[1]) - uint32_t(omin) : known_range;
     auto rel = [&](U x) { return uint32_t(KD::ord(x, flip, NL) - omin); };
     if (range == 0) {  // all keys equal: in input order
 #pragma unroll

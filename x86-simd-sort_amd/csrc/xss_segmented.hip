@@ -89,7 +89,7 @@ struct xss_segplan {
     uint32_t nmed = 0, nbk = 0, lcap = 0;
     uint32_t *med_list = nullptr, *lseg = nullptr, *lnb = nullptr, *lbb = nullptr, *bk_seg = nullptr;
     void *split = nullptr;
-    uint32_t *bstart = nullptr, *bsize = nullptr, *ctl = nullptr, *big = nullptr, *ksel = nullptr;
+    uint32_t *bstart = nullptr, *bsize = nullptr, *brange = nullptr, *ctl = nullptr, *big = nullptr, *ksel = nullptr;
 };
 
 namespace {
@@ -200,7 +200,7 @@ int ss_run(xss_segplan *p, void *keys, uint32_t *idx32, bool desc, hipStream_t s
         zv = zk + align256(p->n * sizeof(U));
     }
     SsLocalArgs la{};
-    la.bstart = p->bstart; la.bsize = p->bsize; la.bk_seg = p->bk_seg; la.lseg = p->lseg; la.seg_off = p->seg_off;
+    la.bstart = p->bstart; la.bsize = p->bsize; la.brange = p->brange; la.bk_seg = p->bk_seg; la.lseg = p->lseg; la.seg_off = p->seg_off;
     la.idx32_out = idx32; la.flip = flip; la.canon = canon; la.keys_out = keys;
     if (p->nmed) {
         SsLocalArgs m = la;
@@ -213,6 +213,7 @@ int ss_run(xss_segplan *p, void *keys, uint32_t *idx32, bool desc, hipStream_t s
     SsArgs a{};
     a.seg_off = p->seg_off; a.lseg = p->lseg; a.lnb = p->lnb; a.lbb = p->lbb; a.bk_seg = p->bk_seg;
     a.oversample = p->tune.oversample; a.split = p->split; a.bstart = p->bstart; a.bsize = p->bsize;
+    a.brange = p->brange;
     a.ctl = p->ctl; a.big = p->big; a.ksel = p->ksel; a.lcap = p->lcap; a.select = p->op == 2;
     a.flip = flip; a.nan_last = NL;
     {
@@ -444,6 +445,7 @@ int xss_gpu_segplan_create(xss_segplan_t *out, const char *op, const char *key_t
     const size_t o_med = take(med.size() * 4 + 4), o_lseg = take(size_t(nbig) * 4 + 4), o_lnb = take(size_t(nbig) * 4 + 4);
     const size_t o_lbb = take(size_t(nbig) * 4 + 4), o_bks = take(nbk * 4 + 4), o_split = take(size_t(nbig) * SS_SPLIT * 4 + 4);
     const size_t o_bst = take(nbk * 4 + 4), o_bsz = take(nbk * 4 + 4), o_big = take(nbk * 4 + 4);
+    const size_t o_brg = take(nbk * 8 + 8);
     const size_t o_ksel = take(size_t(nbig) * 4 + 4), o_ctl = take(16);
     XSS_HIP_CHECK(hipGetDevice(&p->device));
     XSS_HIP_CHECK(hipMalloc(&p->mem, off));
@@ -467,6 +469,7 @@ int xss_gpu_segplan_create(xss_segplan_t *out, const char *op, const char *key_t
     p->split = m + o_split;
     p->bstart = reinterpret_cast<uint32_t *>(m + o_bst);
     p->bsize = reinterpret_cast<uint32_t *>(m + o_bsz);
+    p->brange = reinterpret_cast<uint32_t *>(m + o_brg);
     p->big = reinterpret_cast<uint32_t *>(m + o_big);
     p->ksel = reinterpret_cast<uint32_t *>(m + o_ksel);
     p->ctl = reinterpret_cast<uint32_t *>(m + o_ctl);

@@ -276,12 +276,9 @@ def segmented_batch(torch, xss_amd, L, tuning, reps=3):
         return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
     def plan(op, tag):
-        p = ctypes.c_void_p()
-        rc = L.xss_gpu_segplan_create(ctypes.byref(p), op.encode(), tag.encode(), offp, ctypes.c_size_t(nseg),
-                                      ctypes.c_uint32(tuning.get("small_max", 0)),
-                                      ctypes.c_uint32(tuning.get("range_tiles", 0)))
-        assert rc == 0, L.xss_gpu_last_error()
-        return p
+        """the plan with the autotuner's setting for (op, key type) (tuning.json)"""
+        t = {k: v for k, v in tuning.get(f"{op}_{tag}", {}).items() if k not in ("ms", "sweep")}
+        return xss_amd.SegPlan(op, tag, off, t)
 
     def timed(fn, src=None):
         ts = []
@@ -306,10 +303,12 @@ def segmented_batch(torch, xss_amd, L, tuning, reps=3):
         return bad == 0
 
     out = {"workload": "8192 f32 arrays, len log-uniform [16, 2^20], U(-1,1) with 0.1% NaN, hasnan=true",
-           "n_keys": n, "n_segments": nseg, "tuning": tuning}
+           "n_keys": n, "n_segments": nseg,
+           "tuning": {k: {x: y for x, y in v.items() if x != "sweep"} if isinstance(v, dict) else v
+                      for k, v in tuning.items()}}
     p = plan("qsort", "f32")
     ptr = ctypes.c_void_p(work.data_ptr())
-    run = lambda: L.xss_gpu_segplan_run(p, ptr, None, 1, 0, cur())  # noqa: E731
+    run = lambda: p.run(work.data_ptr(), None, True)  # noqa: E731
     run()
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
@@ -322,11 +321,11 @@ def segmented_batch(torch, xss_amd, L, tuning, reps=3):
     out.update({"segmented_ms": round(t_graph, 3), "loop_ms": round(t_loop, 3),
                 "speedup_vs_loop": round(t_loop / t_graph, 2), "segmented_Gkeys_s": round(n / t_graph / 1e6, 2),
                 "sorted_ok": ok})
-    L.xss_gpu_segplan_destroy(p)
+    p.close()
     # argsort of int32 keys with uint32 indices
     idx = torch.empty(n, dtype=torch.int32, device="cuda")
     p = plan("argsort", "i32")
-    ia = lambda: L.xss_gpu_segplan_run(p, ptr, ctypes.c_void_p(idx.data_ptr()), 0, 0, cur())  # noqa: E731
+    ia = lambda: p.run(work.data_ptr(), idx.data_ptr(), False)  # noqa: E731
     work.copy_(ints)
     ia()
     t_arg = timed(ia, ints)
@@ -335,14 +334,14 @@ def segmented_batch(torch, xss_amd, L, tuning, reps=3):
                                                           ctypes.cast(ctypes.c_void_p(idx64.data_ptr()),
                                                                       ctypes.POINTER(ctypes.c_uint64)),
                                                           offp, ctypes.c_size_t(nseg), cur()), ints)
-    L.xss_gpu_segplan_destroy(p)
+    p.close()
     p = plan("qselect", "f32")
-    sel = lambda: L.xss_gpu_segplan_run(p, ptr, None, 1, 0, cur())  # noqa: E731
+    sel = lambda: p.run(work.data_ptr(), None, True)  # noqa: E731
     sel()
     t_sel = timed(sel)
     t_sel_loop = timed(lambda: L.xss_gpu_loop_qselect_f32(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_float)), offp,
                                                           ctypes.c_size_t(nseg), 1, cur()))
-    L.xss_gpu_segplan_destroy(p)
+    p.close()
     out.update({"argsort_i32_ms": round(t_arg, 3), "argsort_i32_loop_ms": round(t_arg_loop, 3),
                 "qselect_ms": round(t_sel, 3), "qselect_loop_ms": round(t_sel_loop, 3)})
     # the CPU loop's sample: the first 1024 arrays as the plan sorted them
@@ -519,7 +518,7 @@ def main():
         tuning = {}
         tp = os.path.join(REPO, "x86-simd-sort_amd", "tuning.json")
         if os.path.exists(tp):
-            tuning = json.load(open(tp)).get("segmented_f32", {})
+            tuning = json.load(open(tp))
         line["segmented"], seg_extra = segmented_batch(torch, xss_amd, L, tuning)
         line["segmented_speedup_vs_loop"] = line["segmented"]["speedup_vs_loop"]
     if not args.no_serve:

@@ -165,6 +165,54 @@ def object_sort_by_keys(objs, obj_bytes, keys: DArray):
 
 # ---------------------------------------------------------------- serving form
 
+# plan knobs of the segmented form (xss_gpu_set_tuning names; read when a
+# plan is created) besides the two create() arguments
+SEG_KNOBS = ("seg_bucket_keys", "seg_oversample", "seg_local_items", "seg_wide_tiles", "seg_sample_off")
+
+
+class SegPlan:
+    """A segmented plan (xss_gpu_segplan_*): op "qsort" / "argsort" /
+    "qselect" over the segments offsets[i] .. offsets[i + 1] (host uint64)
+    of one device buffer of key type `tag`.  `tuning` (optional dict, e.g. a
+    tuning.json entry) sets small_max, range_tiles and the SEG_KNOBS while
+    the plan is created."""
+
+    def __init__(self, op, tag, offsets, tuning=None):
+        L = lib()
+        t = dict(tuning or {})
+        self._off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self.nseg = len(self._off) - 1
+        for k in SEG_KNOBS:
+            if t.get(k):
+                _check(L.xss_gpu_set_tuning(k.encode(), ctypes.c_uint64(int(t[k]))), "xss_gpu_set_tuning")
+        self.p = ctypes.c_void_p()
+        try:
+            _check(L.xss_gpu_segplan_create(ctypes.byref(self.p), op.encode(), tag.encode(),
+                                            self._off.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                            ctypes.c_size_t(self.nseg), ctypes.c_uint32(int(t.get("small_max", 0))),
+                                            ctypes.c_uint32(int(t.get("range_tiles", 0)))), "xss_gpu_segplan_create")
+        finally:
+            for k in SEG_KNOBS:
+                if t.get(k):
+                    L.xss_gpu_set_tuning(k.encode(), ctypes.c_uint64(0))
+
+    def run(self, keys_ptr, idx_ptr=None, hasnan=False, descending=False):
+        """Enqueue the plan on torch's current stream (device pointers)."""
+        _check(lib().xss_gpu_segplan_run(self.p, ctypes.c_void_p(keys_ptr), ctypes.c_void_p(idx_ptr or 0),
+                                         int(hasnan), int(descending), _stream()), "xss_gpu_segplan_run")
+
+    def close(self):
+        if self.p:
+            lib().xss_gpu_segplan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Request(ctypes.Structure):
     """struct xss_request of include/xss_gpu.h."""
     _fields_ = [("op", ctypes.c_int), ("key_type", ctypes.c_int), ("val_type", ctypes.c_int),

@@ -65,13 +65,16 @@ METRIC = "Gkeys/s + % HBM roofline at 256M keys; segmented-batch vs per-call spe
 
 
 def device_input(torch, dtype, n, seed):
+    """n keys uniform over the 32-bit patterns; for f32, uniform over the
+    finite ones (an infinity or NaN pattern is drawn again)."""
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
     x = torch.randint(-2 ** 31, 2 ** 31, (n,), dtype=torch.int32, device="cuda", generator=g)
     if dtype == "f32":
-        # keep floats finite: exponent all-ones -> flip one exponent bit
-        bad = (x & 0x7F800000) == 0x7F800000
-        x = torch.where(bad, x ^ 0x40000000, x)
+        bad = ((x & 0x7F800000) == 0x7F800000).nonzero().flatten()
+        while bad.numel():
+            x[bad] = torch.randint(-2 ** 31, 2 ** 31, (bad.numel(),), dtype=torch.int32, device="cuda", generator=g)
+            bad = bad[(x[bad] & 0x7F800000) == 0x7F800000]
     return x
 
 

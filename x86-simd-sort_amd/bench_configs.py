@@ -71,22 +71,28 @@ def torch_dtype(torch, tag):
 
 
 def rand_bits(torch, tag, n, g):
-    """Uniform over the type's bit patterns (finite for floats), as raw integers."""
+    """Uniform over the type's bit patterns, as raw integers; for floats,
+    uniform over the finite patterns (an infinity or NaN pattern is drawn
+    again until it is finite)."""
     kb = BYTES[tag]
-    if kb == 2:
-        x = torch.randint(-2 ** 15, 2 ** 15, (n,), dtype=torch.int16, device="cuda", generator=g)
-        exp, flip = 0x7C00, 0x4000
-    elif kb == 4:
-        x = torch.randint(-2 ** 31, 2 ** 31, (n,), dtype=torch.int32, device="cuda", generator=g)
-        exp, flip = 0x7F800000, 0x40000000
-    else:
-        hi = torch.randint(-2 ** 31, 2 ** 31, (n,), dtype=torch.int64, device="cuda", generator=g)
-        lo = torch.randint(0, 2 ** 32, (n,), dtype=torch.int64, device="cuda", generator=g)
-        x = (hi << 32) | lo
-        exp, flip = 0x7FF0000000000000, 0x4000000000000000
+
+    def draw(m):
+        if kb == 2:
+            return torch.randint(-2 ** 15, 2 ** 15, (m,), dtype=torch.int16, device="cuda", generator=g)
+        if kb == 4:
+            return torch.randint(-2 ** 31, 2 ** 31, (m,), dtype=torch.int32, device="cuda", generator=g)
+        hi = torch.randint(-2 ** 31, 2 ** 31, (m,), dtype=torch.int64, device="cuda", generator=g)
+        lo = torch.randint(0, 2 ** 32, (m,), dtype=torch.int64, device="cuda", generator=g)
+        return (hi << 32) | lo
+
+    x = draw(n)
     if tag in ("f16", "f32", "f64"):
+        exp = {2: 0x7C00, 4: 0x7F800000, 8: 0x7FF0000000000000}[kb]
         e = torch.tensor(exp, dtype=x.dtype, device="cuda")
-        x = torch.where((x & e) == e, x ^ flip, x)
+        bad = ((x & e) == e).nonzero().flatten()
+        while bad.numel():
+            x[bad] = draw(bad.numel())
+            bad = bad[(x[bad] & e) == e]
     return x
 
 

@@ -18,17 +18,22 @@
 //   ss_scatter  the scatter of a segmented pass with the bucket as the digit
 //               (keys -> scratch; argsort generates its indices here)
 //   ss_buckets  bucket starts and sizes; buckets too large for the LDS sort
-//               are listed (select: only the bucket holding rank len / 2)
+//               are listed (select: only the bucket holding rank len / 2,
+//               whose slot is kept per segment)
 //   ss_local    one workgroup per bucket sorts it by its whole order key in
 //               LDS: the bits where its keys differ (AND / OR of the bucket),
 //               a 12-bit pass whose order inside a digit is free, then
 //               stable 4-bit passes; argsort: stable 4-bit passes only
 //   ss_big      listed buckets: one workgroup runs 8-bit LSD passes over
 //               the bucket in HBM (digits its keys share are skipped)
+//   select      the histogram read also copies the keys to scratch;
+//               ss_part3 writes them back in three regions (below, inside,
+//               above the bucket holding rank len / 2) and only that bucket
+//               is sorted, in place
 //
 // Moved bytes per key: a histogram read, one scatter (read + write) and the
 // LDS sort's read + write: 20 bytes for 4-byte keys, against 36 for four LSD
-// passes.  Everything is planned on the host (segment lengths) or on the
+// passes; select: 16 (read, copy, partition read + write).  Everything is planned on the host (segment lengths) or on the
 // device (bucket sizes), with no host wait: a plan's run is capturable.
 #pragma once
 #include "xss_sort.hpp"
@@ -177,7 +182,7 @@ struct SsRanges {
 // One workgroup per range: the bucket counts of its keys (rsum[d][range]).
 template <class KD>
 __global__ __launch_bounds__(256) void ss_hist_kernel(SsArgs a, SsRanges r, const void *keys_v, uint32_t *rsum,
-                                                      uint32_t stride)
+                                                      uint32_t stride, void *copy_v)
 {
     using U = typename KD::U;
     __shared__ U sp[SS_SPLIT];
@@ -210,6 +215,9 @@ __global__ __launch_bounds__(256) void ss_hist_kernel(SsArgs a, SsRanges r, cons
     const uint64_t v0 = r0 + head;
     const uint64_t nvec = (r1 - v0) / VEC;
     const uint4 *kv = reinterpret_cast<const uint4 *>(keys + v0);
+    // copy_v (select): the keys are also copied to the same positions there
+    U *copy = static_cast<U *>(copy_v);
+    uint4 *cv = copy ? reinterpret_cast<uint4 *>(copy + v0) : nullptr;
     constexpr uint32_t UNR = 2;
     uint64_t v = tid;
     for (; v + (UNR - 1) * 256 < nvec; v += UNR * 256) {
@@ -218,6 +226,7 @@ __global__ __launch_bounds__(256) void ss_hist_kernel(SsArgs a, SsRanges r, cons
         for (uint32_t u = 0; u < UNR; ++u) q[u] = kv[v + u * 256];
 #pragma unroll
         for (uint32_t u = 0; u < UNR; ++u) {
+            if (cv) cv[v + u * 256] = q[u];
             const U *e = reinterpret_cast<const U *>(&q[u]);
 #pragma unroll
             for (int k = 0; k < VEC; ++k) count(e[k]);
@@ -225,13 +234,19 @@ __global__ __launch_bounds__(256) void ss_hist_kernel(SsArgs a, SsRanges r, cons
     }
     for (; v < nvec; v += 256) {
         const uint4 q = kv[v];
+        if (cv) cv[v] = q;
         const U *e = reinterpret_cast<const U *>(&q);
 #pragma unroll
         for (int k = 0; k < VEC; ++k) count(e[k]);
     }
     const uint64_t t0 = v0 + nvec * VEC;
     const uint64_t rest = head + (r1 - t0);
-    for (uint64_t i = tid; i < rest; i += 256) count(keys[i < head ? r0 + i : t0 + (i - head)]);
+    for (uint64_t i = tid; i < rest; i += 256) {
+        const uint64_t p = i < head ? r0 + i : t0 + (i - head);
+        const U x = keys[p];
+        if (copy) copy[p] = x;
+        count(x);
+    }
     __syncthreads();
     rsum[size_t(tid) * stride + rg] = h[tid];
 }
@@ -658,6 +673,7 @@ struct SsBigArgs {
     uint32_t *idx32_out;       // argsort
     uint64_t flip;
     uint32_t canon;
+    uint32_t end_in_y;         // the keys end in y (select: y is the caller's array)
 };
 
 // One workgroup per listed bucket: 8-bit LSD passes through HBM over the
@@ -736,17 +752,17 @@ __global__ __launch_bounds__(SS_BIG_BLOCK) void ss_big_kernel(SsBigArgs g)
             dv = tv;
             in_y = !in_y;
         }
-        // keys left in the scatter's buffer (an even number of passes, or
-        // none): copied to the caller's array (argsort: the indices written)
+        // keys left in the other buffer than the one they end in (by the
+        // parity of the passes): copied (argsort: the indices written)
         if constexpr (IDX) {
             if (dmask == 0) {
                 const V *yv = static_cast<const V *>(g.y_vals);
                 for (uint64_t p = st + tid; p < st + m; p += BLOCK) g.idx32_out[p] = yv[p] - uint32_t(segb);
             }
         } else {
-            if (in_y) {
-                const U *src = static_cast<const U *>(g.y_keys);
-                U *dst = static_cast<U *>(g.z_keys);
+            if (in_y != (g.end_in_y != 0)) {
+                const U *src = static_cast<const U *>(in_y ? g.y_keys : g.z_keys);
+                U *dst = static_cast<U *>(in_y ? g.z_keys : g.y_keys);
                 for (uint64_t p = st + tid; p < st + m; p += BLOCK) {
                     U x = src[p];
                     if constexpr (KD::KIND == Kind::Float && NL)
@@ -760,13 +776,106 @@ __global__ __launch_bounds__(SS_BIG_BLOCK) void ss_big_kernel(SsBigArgs g)
 
 // ---------------------------------------------------------------- select
 
-// Copy of [begin, end) of every range (the select's scatter output back to
-// the caller's keys; the bucket holding rank len / 2 is then sorted over it).
-template <class U>
-__global__ __launch_bounds__(256) void ss_copy_kernel(SsRanges r, const U *src, U *dst)
+// Select's partition: every key of a large segment goes to one of three
+// regions of the caller's array -- below, inside or above the bucket that
+// holds rank len / 2 (order-key bounds from its splitters) -- from the copy
+// the histogram read made.  One workgroup per range (4 waves); a tile of
+// 16 keys per thread is classified, each key ranked inside its wave and
+// class by ballots (no atomics), the waves' class counts scanned in LDS,
+// and every key stored at its region's running base.  The order inside a
+// region is free.  Range bases come from the bucket histogram's scan: the
+// region below starts at the segment's start plus the range's counts of
+// the lower buckets in earlier ranges, and so on.
+constexpr int SS_P3_BLOCK = 256;
+constexpr int SS_P3_ITEMS = 16;
+template <class KD, bool NL>
+__global__ __launch_bounds__(SS_P3_BLOCK) void ss_part3_kernel(SsArgs a, SegRanges r, const void *src_v,
+                                                                void *dst_v)
 {
-    const uint64_t r0 = r.begin[blockIdx.x], r1 = r.end[blockIdx.x];
-    for (uint64_t p = r0 + threadIdx.x; p < r1; p += 256) dst[p] = src[p];
+    using U = typename KD::U;
+    constexpr int WAVES = SS_P3_BLOCK / 64;
+    constexpr uint32_t TILE = SS_P3_BLOCK * SS_P3_ITEMS;
+    __shared__ uint32_t s_red[3][256];
+    __shared__ uint32_t s_wc[WAVES][3];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, b = blockIdx.x;
+    const uint32_t j = r.rbseg[b];
+    const uint32_t kd = a.ksel[j] - a.lbb[j], nb = a.lnb[j];
+    const U *sp = static_cast<const U *>(a.split) + size_t(j) * SS_SPLIT;
+    const U lo = kd > 0 ? sp[kd - 1] : U(0);
+    const U hi = kd + 1 < nb ? U(sp[kd] - 1) : U(KD::ONES);  // inclusive
+    // region bases of this range: segment totals and the range's prefix
+    // (digit-major rpre) summed below / at / above kd
+    const uint32_t tot = r.segtot[size_t(j) * 256 + tid];
+    const uint32_t pre = r.rpre[size_t(tid) * r.stride + b];
+    s_red[0][tid] = tid < kd ? tot : 0u;
+    s_red[1][tid] = tid < kd ? pre : 0u;
+    s_red[2][tid] = tid > kd ? pre : 0u;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+            s_red[0][tid] += s_red[0][tid + h];
+            s_red[1][tid] += s_red[1][tid + h];
+            s_red[2][tid] += s_red[2][tid + h];
+        }
+        __syncthreads();
+    }
+    const uint64_t sbase = r.sbase[b];
+    const uint32_t l_lt = s_red[0][0], l_eq = r.segtot[size_t(j) * 256 + kd];
+    uint64_t base[3] = {sbase + s_red[1][0], sbase + l_lt + r.rpre[size_t(kd) * r.stride + b],
+                        sbase + l_lt + l_eq + s_red[2][0]};
+    const U *src = static_cast<const U *>(src_v);
+    U *dst = static_cast<U *>(dst_v);
+    const U flip = U(a.flip);
+    const uint64_t rb = r.begin[b], re = r.end[b];
+    for (uint64_t t = rb; t < re; t += TILE) {
+        U key[SS_P3_ITEMS];
+        uint32_t off[SS_P3_ITEMS];  // class (bits 30..31), place in the wave's class run of this tile
+        uint32_t wc0 = 0, wc1 = 0, wc2 = 0;
+#pragma unroll
+        for (int i = 0; i < SS_P3_ITEMS; ++i) {
+            const uint64_t p = t + uint64_t(i) * SS_P3_BLOCK + tid;
+            const bool real = p < re;
+            key[i] = real ? src[p] : U(0);
+            const U o = KD::ord(key[i], flip, NL);
+            const uint32_t c = o < lo ? 0u : (o <= hi ? 1u : 2u);
+            const uint64_t m0 = __ballot(real && c == 0), m1 = __ballot(real && c == 1), m2 = __ballot(real && c == 2);
+            const uint64_t mine = c == 0 ? m0 : c == 1 ? m1 : m2;
+            const uint32_t run = c == 0 ? wc0 : c == 1 ? wc1 : wc2;
+            off[i] = real ? ((c << 30) | (run + lanes_below(mine))) : ~0u;
+            wc0 += uint32_t(__popcll(m0));
+            wc1 += uint32_t(__popcll(m1));
+            wc2 += uint32_t(__popcll(m2));
+        }
+        if (lane == 0) {
+            s_wc[wave][0] = wc0;
+            s_wc[wave][1] = wc1;
+            s_wc[wave][2] = wc2;
+        }
+        __syncthreads();
+        uint64_t wb[3];
+        uint32_t total[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            uint32_t before = 0, all = 0;
+            for (int w = 0; w < WAVES; ++w) {
+                const uint32_t x = s_wc[w][c];
+                before += w < int(wave) ? x : 0u;
+                all += x;
+            }
+            wb[c] = base[c] + before;
+            total[c] = all;
+        }
+#pragma unroll
+        for (int i = 0; i < SS_P3_ITEMS; ++i) {
+            if (off[i] != ~0u) {
+                const uint32_t c = off[i] >> 30;
+                dst[wb[c] + (off[i] & 0x3fffffffu)] = key[i];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) base[c] += total[c];
+        __syncthreads();  // s_wc is read before the next tile writes it
+    }
 }
 
 }  // namespace xss

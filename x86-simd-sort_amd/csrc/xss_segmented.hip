@@ -224,8 +224,11 @@ int ss_run(xss_segplan *p, void *keys, uint32_t *idx32, bool desc, hipStream_t s
     XSS_HIP_CHECK(hipGetLastError());
     {
         KernelTimer t("ss_hist", s);
+        // select: the histogram read also copies the keys to scratch, from
+        // where the partition writes them back
         hipLaunchKernelGGL((ss_hist_kernel<KD>), dim3(p->nr), dim3(256), 0, s, a, SsRanges{p->r_begin, p->r_end, p->r_bseg},
-                           static_cast<const void *>(keys), p->rsum, p->nr);
+                           static_cast<const void *>(keys), p->rsum, p->nr,
+                           p->op == 2 ? static_cast<void *>(yk) : nullptr);
     }
     XSS_HIP_CHECK(hipGetLastError());
     SegScanArgs sc{p->rsum, p->rpre, p->segtot, p->r_head, p->r_bseg, p->nr, p->nr, nullptr};
@@ -234,7 +237,7 @@ int ss_run(xss_segplan *p, void *keys, uint32_t *idx32, bool desc, hipStream_t s
         hipLaunchKernelGGL(seg_scan_kernel<0>, dim3(256), dim3(256), 0, s, sc);
     }
     XSS_HIP_CHECK(hipGetLastError());
-    {
+    if (p->op != 2) {
         PassArgs pa{};
         pa.keys_in = keys; pa.keys_out = yk; pa.vals_out = yv; pa.n = uint32_t(p->n); pa.flip = flip;
         pa.nan_last = NL; pa.first = 1; pa.last = 0;
@@ -259,13 +262,16 @@ int ss_run(xss_segplan *p, void *keys, uint32_t *idx32, bool desc, hipStream_t s
     XSS_HIP_CHECK(hipGetLastError());
     la.keys_in = yk; la.vals_in = yv;
     if (p->op == 2) {
-        // select: the partition back in the caller's keys, then the bucket
-        // holding rank len / 2 of every large segment sorted over it
-        KernelTimer t("ss_copy", s);
-        hipLaunchKernelGGL(ss_copy_kernel<U>, dim3(p->nr), dim3(256), 0, s, SsRanges{p->r_begin, p->r_end, p->r_bseg},
-                           reinterpret_cast<const U *>(yk), static_cast<U *>(keys));
+        // select: the copy partitioned back into the caller's keys (below /
+        // inside / above the bucket holding rank len / 2), then that bucket
+        // of every large segment sorted in place
+        SegRanges r{p->r_begin, p->r_end, p->r_sbase, p->r_bseg, p->rpre, p->segtot, p->nr, p->nr, nullptr};
+        KernelTimer t("ss_part3", s);
+        hipLaunchKernelGGL((ss_part3_kernel<KD, NL>), dim3(p->nr), dim3(SS_P3_BLOCK), 0, s, a, r,
+                           static_cast<const void *>(yk), keys);
         XSS_HIP_CHECK(hipGetLastError());
         la.slots = p->ksel;
+        la.keys_in = keys;
     }
     {
         KernelTimer t("ss_local", s);
@@ -278,6 +284,11 @@ int ss_run(xss_segplan *p, void *keys, uint32_t *idx32, bool desc, hipStream_t s
         g.ctl = p->ctl; g.big = p->big; g.bstart = p->bstart; g.bsize = p->bsize; g.bk_seg = p->bk_seg; g.lseg = p->lseg;
         g.seg_off = p->seg_off; g.y_keys = yk; g.y_vals = yv; g.z_keys = zk; g.z_vals = zv; g.idx32_out = idx32;
         g.flip = flip; g.canon = canon;
+        if (p->op == 2) {  // select: the bucket is in the caller's keys and ends there
+            g.y_keys = keys;
+            g.z_keys = yk;
+            g.end_in_y = 1;
+        }
         KernelTimer t("ss_big", s);
         hipLaunchKernelGGL((ss_big_kernel<KD, PL, NL>), dim3(std::min<uint32_t>(p->nbk, 256)), dim3(SS_BIG_BLOCK), 0, s,
                            g);

@@ -782,8 +782,8 @@ __global__ __launch_bounds__(SS_BIG_BLOCK) void ss_big_kernel(SsBigArgs g)
 // the histogram read made.  One workgroup per range (4 waves); a tile of
 // 16 keys per thread is classified, each key ranked inside its wave and
 // class by ballots (no atomics), the waves' class counts scanned in LDS,
-// and every key stored at its region's running base.  The order inside a
-// region is free.  Range bases come from the bucket histogram's scan: the
+// the tile laid out in LDS by class, and written out as three contiguous
+// runs at the regions' running bases.  The order inside a region is free.  Range bases come from the bucket histogram's scan: the
 // region below starts at the segment's start plus the range's counts of
 // the lower buckets in earlier ranges, and so on.
 constexpr int SS_P3_BLOCK = 256;
@@ -797,6 +797,7 @@ __global__ __launch_bounds__(SS_P3_BLOCK) void ss_part3_kernel(SsArgs a, SegRang
     constexpr uint32_t TILE = SS_P3_BLOCK * SS_P3_ITEMS;
     __shared__ uint32_t s_red[3][256];
     __shared__ uint32_t s_wc[WAVES][3];
+    __shared__ U s_tile[TILE];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, b = blockIdx.x;
     const uint32_t j = r.rbseg[b];
     const uint32_t kd = a.ksel[j] - a.lbb[j], nb = a.lnb[j];
@@ -852,8 +853,9 @@ __global__ __launch_bounds__(SS_P3_BLOCK) void ss_part3_kernel(SsArgs a, SegRang
             s_wc[wave][2] = wc2;
         }
         __syncthreads();
-        uint64_t wb[3];
-        uint32_t total[3];
+        // tile slots: class c starts at cs[c]; wave w's keys of class c
+        // after those of the waves before it
+        uint32_t wslot[3], total[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             uint32_t before = 0, all = 0;
@@ -862,19 +864,29 @@ __global__ __launch_bounds__(SS_P3_BLOCK) void ss_part3_kernel(SsArgs a, SegRang
                 before += w < int(wave) ? x : 0u;
                 all += x;
             }
-            wb[c] = base[c] + before;
+            wslot[c] = before;
             total[c] = all;
         }
+        const uint32_t cs1 = total[0], cs2 = total[0] + total[1], n_tile = cs2 + total[2];
+        wslot[1] += cs1;
+        wslot[2] += cs2;
 #pragma unroll
         for (int i = 0; i < SS_P3_ITEMS; ++i) {
             if (off[i] != ~0u) {
                 const uint32_t c = off[i] >> 30;
-                dst[wb[c] + (off[i] & 0x3fffffffu)] = key[i];
+                s_tile[wslot[c] + (off[i] & 0x3fffffffu)] = key[i];
             }
+        }
+        __syncthreads();
+        // write-out: consecutive threads take consecutive slots, so each
+        // class is one contiguous run in HBM
+        for (uint32_t q = tid; q < n_tile; q += SS_P3_BLOCK) {
+            const uint64_t g = q < cs1 ? base[0] + q : q < cs2 ? base[1] + (q - cs1) : base[2] + (q - cs2);
+            dst[g] = s_tile[q];
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) base[c] += total[c];
-        __syncthreads();  // s_wc is read before the next tile writes it
+        __syncthreads();  // s_wc and s_tile are read before the next tile writes them
     }
 }
 

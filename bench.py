@@ -252,7 +252,7 @@ def cpu_baseline(torch, L, dtype, gpu_ms_headline, seg=None, quick=False):
     return out
 
 
-def segmented_batch(torch, xss_amd, L, tuning, reps=3):
+def segmented_batch(torch, xss_amd, L, tuning, reps=3, loops=True):
     """8192 float32 arrays, lengths log-uniform in [16, 2^20] (seeded), values
     U(-1, 1) with 0.1% NaN, hasnan = true: one plan captured in a graph and
     replayed, against a C++ loop of plain xss_gpu_qsort_f32 calls on the same
@@ -312,7 +312,8 @@ def segmented_batch(torch, xss_amd, L, tuning, reps=3):
     p = plan("qsort", "f32")
     ptr = ctypes.c_void_p(work.data_ptr())
     run = lambda: p.run(work.data_ptr(), None, True)  # noqa: E731
-    run()
+    work.copy_(pristine)
+    run()  # warm-up on the real input (outside capture)
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
@@ -320,7 +321,7 @@ def segmented_batch(torch, xss_amd, L, tuning, reps=3):
     t_graph = timed(graph.replay)
     ok = check_sorted()
     t_loop = timed(lambda: L.xss_gpu_loop_qsort_f32(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_float)), offp,
-                                                    ctypes.c_size_t(nseg), 1, 0, cur()))
+                                                    ctypes.c_size_t(nseg), 1, 0, cur())) if loops else float("nan")
     out.update({"segmented_ms": round(t_graph, 3), "loop_ms": round(t_loop, 3),
                 "speedup_vs_loop": round(t_loop / t_graph, 2), "segmented_Gkeys_s": round(n / t_graph / 1e6, 2),
                 "sorted_ok": ok})
@@ -336,14 +337,16 @@ def segmented_batch(torch, xss_amd, L, tuning, reps=3):
     t_arg_loop = timed(lambda: L.xss_gpu_loop_argsort_i32(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_int32)),
                                                           ctypes.cast(ctypes.c_void_p(idx64.data_ptr()),
                                                                       ctypes.POINTER(ctypes.c_uint64)),
-                                                          offp, ctypes.c_size_t(nseg), cur()), ints)
+                                                          offp, ctypes.c_size_t(nseg), cur()), ints) \
+        if loops else float("nan")
     p.close()
     p = plan("qselect", "f32")
     sel = lambda: p.run(work.data_ptr(), None, True)  # noqa: E731
+    work.copy_(pristine)
     sel()
     t_sel = timed(sel)
     t_sel_loop = timed(lambda: L.xss_gpu_loop_qselect_f32(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_float)), offp,
-                                                          ctypes.c_size_t(nseg), 1, cur()))
+                                                          ctypes.c_size_t(nseg), 1, cur())) if loops else float("nan")
     p.close()
     out.update({"argsort_i32_ms": round(t_arg, 3), "argsort_i32_loop_ms": round(t_arg_loop, 3),
                 "qselect_ms": round(t_sel, 3), "qselect_loop_ms": round(t_sel_loop, 3)})
@@ -382,6 +385,8 @@ def main():
     ap.add_argument("--no-seg", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--quick-cpu", action="store_true", help="smaller CPU samples (a quick check of the legs)")
+    ap.add_argument("--seg-only", action="store_true",
+                    help="only the segmented plans (no loop baselines; for profiling their kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X, and the sort of one array "
@@ -392,6 +397,12 @@ def main():
     import xss_amd
     L = xss_amd.lib()
     torch.cuda.set_device(0)
+    if args.seg_only:
+        tp = os.path.join(REPO, "x86-simd-sort_amd", "tuning.json")
+        tuning = json.load(open(tp)) if os.path.exists(tp) else {}
+        seg, _ = segmented_batch(torch, xss_amd, L, tuning, loops=False)
+        print(json.dumps({k: v for k, v in seg.items() if k != "tuning"}))
+        return
     n, dt = args.n, args.dtype
     key_bytes = 4
     pristine = device_input(torch, dt, n, 1234)

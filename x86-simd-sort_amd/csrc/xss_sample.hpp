@@ -9,8 +9,9 @@
 // every key is moved once through HBM and then sorted on chip:
 //
 //   ss_sample   per segment: S stratified samples (S = a power of two, about
-//               `oversample` per bucket), bitonic-sorted in LDS by order key;
-//               nb - 1 evenly spaced ones are the splitters
+//               `oversample` per bucket; in groups of 16 consecutive keys,
+//               one line each), bitonic-sorted in LDS by order key; nb - 1
+//               evenly spaced ones are the splitters
 //   ss_hist     per range (ranges never cross a segment): the bucket of each
 //               key (binary search over the segment's splitters in LDS),
 //               counted into 256 bins -- the digit histogram of a pass
@@ -145,10 +146,15 @@ __global__ __launch_bounds__(SS_SAMPLE_BLOCK) void ss_sample_kernel(SsArgs a, co
     const U *keys = static_cast<const U *>(keys_v) + b0;
     const U flip = U(a.flip);
     const bool nl = a.nan_last != 0;
-    // one sample per stratum [i len / S, (i + 1) len / S), at a hashed offset
+    // S / G groups of G consecutive keys (one 64-byte line), one group per
+    // stratum [g len / (S / G), (g + 1) len / (S / G)) at a hashed offset:
+    // single-key samples would fetch a line each
+    constexpr uint32_t G = 64 / sizeof(U);
+    const uint32_t ng = S >= G ? S / G : 1, gl = S >= G ? G : S;
     for (uint32_t i = tid; i < S; i += SS_SAMPLE_BLOCK) {
-        const uint64_t lo = uint64_t(i) * len / S, hi = uint64_t(i + 1) * len / S;
-        const uint64_t pos = lo + ss_hash(i * 0x9e3779b9u ^ (j + 1) * 0x85ebca6bu) % (hi - lo);
+        const uint32_t gi = i / gl, e = i % gl;
+        const uint64_t lo = uint64_t(gi) * len / ng, hi = uint64_t(gi + 1) * len / ng;  // hi - lo >= gl
+        const uint64_t pos = lo + ss_hash(gi * 0x9e3779b9u ^ (j + 1) * 0x85ebca6bu) % (hi - lo - gl + 1) + e;
         s[i] = KD::ord(keys[pos], flip, nl);
     }
     __syncthreads();
@@ -365,8 +371,9 @@ __device__ __forceinline__ uint64_t match_any4(uint32_t d)
 // free), which hold about one key each for spread data.  Every key's final
 // place is then its sub-bucket's start plus the number of keys of the
 // sub-bucket that order before it (by order key, ties by slot -- argsort: by
-// original index, so equal keys keep the input order); it is written
-// straight to its place in HBM.
+// original index, so equal keys keep the input order); an inverse map in LDS
+// (the slot that ends at each place) then lets the workgroup write the
+// bucket out in place order, so the stores are contiguous.
 //
 // A sub-bucket of more than SS_SUB_MAX keys (duplicates, or keys crowded in
 // a small part of the range) would make the comparisons quadratic; the
@@ -394,6 +401,7 @@ void ss_local_kernel(SsLocalArgs a)
     __shared__ U s_keys[CAP];
     __shared__ V s_vals[IDX ? CAP : 1];
     __shared__ uint32_t s_cnt[2048];  // 4096 16-bit counters / sub-bucket starts
+    __shared__ uint16_t s_inv[CAP];   // rank path: the slot whose key ends at each position
     __shared__ uint32_t s_wc[WAVES * 16];
     __shared__ uint32_t s_wsum[WAVES];
     __shared__ uint32_t s_ao[3];
@@ -583,8 +591,14 @@ void ss_local_kernel(SsLocalArgs a)
                     }
                     r += before ? 1u : 0u;
                 }
-                put(lo + r, x, vx);
+                s_inv[lo + r] = uint16_t(p);
             }
+        }
+        __syncthreads();
+        // written in position order: consecutive threads, consecutive places
+        for (uint32_t q = tid; q < m; q += BLOCK) {
+            const uint32_t src = s_inv[q];
+            put(q, s_keys[src], IDX ? s_vals[src] : V(0));
         }
         return;
     }

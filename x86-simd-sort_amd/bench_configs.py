@@ -19,8 +19,11 @@ c6  (with --only c6) the PyTorch shim torch.ops.xss.sort beside torch.sort:
 Each measurement restores the input from a pristine device copy outside the
 timed region and times the call alone with HIP events on the stream it runs
 on; the median of --reps is reported.  One JSON line per measurement with
-Gkeys/s and the fraction of the 8 TB/s HBM peak at the algorithmic bytes
-per key of the 8-bit LSD plan:
+Gkeys/s, the path the library takes (`path`), the bytes per key that path
+moves (`moved_bytes_per_key`) and the fraction of the 8 TB/s HBM peak they
+reach (`achieved_frac`), beside the fraction at the algorithmic bytes per
+key of the 8-bit LSD plan (`lsd_equiv_frac`, an effective rate: above
+achieved_frac where the path moves fewer bytes):
   sort      s_k (1 + 2P)
   sort      2 s_k for 16-bit keys (the counting sort: one read, one write)
   argsort   s_k + (2 s_k + s_i) + 2 (P - 2)(s_k + s_i) + (s_k + s_i + s_o)
@@ -28,6 +31,14 @@ per key of the 8-bit LSD plan:
   select    4 s_k (+ 2 s_v for key-value)
   object    24 (read xyz) + 8 (write key) + argsort(f64) + 48 (gather + copy back)
 (P passes, s_i = 4-byte internal index, s_o = 8-byte output index).
+Paths and their moved bytes per key:
+  lsd       the plan above
+  count     qsort of 2^23+ 16-bit keys: one read, one write (2 s_k)
+  msd       qsort / argsort / kv sort of 2^25+ 4- or 8-byte keys, MSD-first:
+            a two-digit histogram read, two scatter passes and the LDS bucket
+            sort -- sort 7 s_k, argsort 6 s_k + 24, kv s_k + 6 (s_k + s_v)
+            (passes over constant digits and large-bucket passes not counted)
+  select    radix select: as above
 """
 import argparse
 import ctypes
@@ -137,13 +148,40 @@ def timed(torch, restore, call, reps):
     return statistics.median(ts[1:])  # the first run warms the workspace
 
 
-def emit(cfg, op, tag, n, ms, bpk, **kw):
+HYB_MIN_N, COUNT_MIN_N = 1 << 25, 1 << 23  # the library's default routing thresholds
+
+
+def route(op, kb, n, vb=0):
+    """(path, moved bytes per key) of the call as the library routes it"""
+    if op == "qsort" and kb == 2 and n >= COUNT_MIN_N:
+        return "count", 2 * kb
+    if op in ("qsort", "argsort", "kv") and kb >= 4 and n >= HYB_MIN_N:
+        if op == "qsort":
+            return "msd", 7 * kb
+        if op == "argsort":
+            return "msd", 6 * kb + 24
+        return "msd", kb + 6 * (kb + vb)
+    if op in ("select", "partial"):
+        return "select", alg_bytes(op, kb, vb)
+    return "lsd", alg_bytes(op, kb, vb)
+
+
+def emit(cfg, op, tag, n, ms, bpk, moved=None, path=None, **kw):
     gk = n / (ms * 1e-3) / 1e9
-    ach = n * bpk / (ms * 1e-3) / 1e9
+    eff = n * bpk / (ms * 1e-3) / 1e9
+    moved = bpk if moved is None else moved
+    ach = n * moved / (ms * 1e-3) / 1e9
     rec = {"config": cfg, "op": op, "type": tag, "n": n, "ms": round(ms, 4), "Gkeys_s": round(gk, 2),
-           "alg_bytes_per_key": bpk, "achieved_GBs": round(ach, 1), "hbm_frac": round(ach / PEAK, 4)}
+           "path": path or "lsd", "moved_bytes_per_key": moved, "achieved_GBs": round(ach, 1),
+           "achieved_frac": round(ach / PEAK, 4), "alg_bytes_per_key": bpk, "lsd_equiv_frac": round(eff / PEAK, 4)}
     rec.update(kw)
     print(json.dumps(rec), flush=True)
+
+
+def emit_op(cfg, op, tag, n, ms, kb, vb=0, route_op=None, **kw):
+    """emit with the LSD-plan bytes and the routed path's bytes of op"""
+    path, moved = route(route_op or op, kb, n, vb)
+    emit(cfg, op, tag, n, ms, alg_bytes(route_op or op, kb, vb), moved=moved, path=path, **kw)
 
 
 def c1(torch, L, reps):
@@ -155,7 +193,7 @@ def c1(torch, L, reps):
     ms = timed(torch, lambda: w.copy_(src),
                lambda: L.xss_gpu_qsort_f32(ctypes.c_void_p(w.data_ptr()), ctypes.c_size_t(n), 0, 0, stream(torch)),
                max(reps, 20))
-    emit("c1", "qsort", "f32", n, ms, alg_bytes("qsort", 4))
+    emit_op("c1", "qsort", "f32", n, ms, 4)
 
 
 def c2(torch, L, reps, types=None, recipes=None):
@@ -178,12 +216,12 @@ def c2(torch, L, reps, types=None, recipes=None):
                 ms = timed(torch, lambda: w.copy_(src),
                            lambda: f(ctypes.c_void_p(w.data_ptr()), ctypes.c_size_t(n), hasnan, desc, stream(torch)),
                            reps)
-                emit("c2", "qsort", tag, n, ms, alg_bytes("qsort", kb), recipe=recipe, descending=bool(desc))
+                emit_op("c2", "qsort", tag, n, ms, kb, recipe=recipe, descending=bool(desc))
                 fa = getattr(L, f"xss_gpu_argsort_{tag}")
                 ms = timed(torch, lambda: None,
                            lambda: fa(ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(idx.data_ptr()),
                                       ctypes.c_size_t(n), hasnan, desc, stream(torch)), reps)
-                emit("c2", "argsort", tag, n, ms, alg_bytes("argsort", kb), recipe=recipe, descending=bool(desc))
+                emit_op("c2", "argsort", tag, n, ms, kb, recipe=recipe, descending=bool(desc))
             del src, w
         del idx
         torch.cuda.empty_cache()
@@ -217,26 +255,26 @@ def c3(torch, L, reps):
         pk, pv = ctypes.c_void_p(wk.data_ptr()), ctypes.c_void_p(wv.data_ptr())
         f = getattr(L, f"xss_gpu_keyvalue_qsort_{kt}_{vt}")
         ms = timed(torch, restore, lambda: f(pk, pv, ctypes.c_size_t(n), 0, 0, stream(torch)), reps)
-        emit("c3", "keyvalue_qsort", f"{kt}/{vt}", n, ms, alg_bytes("kv", kb, vb))
+        emit_op("c3", "keyvalue_qsort", f"{kt}/{vt}", n, ms, kb, vb, route_op="kv")
         for k in (n // 2, n - 1):
             f = getattr(L, f"xss_gpu_keyvalue_select_{kt}_{vt}")
             ms = timed(torch, restore, lambda: f(pk, pv, ctypes.c_size_t(k), ctypes.c_size_t(n), 0, 0, stream(torch)),
                        reps)
-            emit("c3", "keyvalue_select", f"{kt}/{vt}", n, ms, alg_bytes("select", kb, vb), k=k)
+            emit("c3", "keyvalue_select", f"{kt}/{vt}", n, ms, alg_bytes("select", kb, vb), path="select", k=k)
             f = getattr(L, f"xss_gpu_qselect_{kt}")
             ms = timed(torch, lambda: wk.copy_(ks),
                        lambda: f(pk, ctypes.c_size_t(k), ctypes.c_size_t(n), 0, 0, stream(torch)), reps)
-            emit("c3", "qselect", kt, n, ms, alg_bytes("select", kb), k=k)
+            emit("c3", "qselect", kt, n, ms, alg_bytes("select", kb), path="select", k=k)
         for k in (1000, n // 100):
             f = getattr(L, f"xss_gpu_keyvalue_partial_sort_{kt}_{vt}")
             ms = timed(torch, restore, lambda: f(pk, pv, ctypes.c_size_t(k), ctypes.c_size_t(n), 0, 0, stream(torch)),
                        reps)
-            emit("c3", "keyvalue_partial_sort", f"{kt}/{vt}", n, ms, alg_bytes("select", kb, vb), k=k,
+            emit("c3", "keyvalue_partial_sort", f"{kt}/{vt}", n, ms, alg_bytes("select", kb, vb), path="select", k=k,
                  note="bytes exclude the sort of the k prefix")
             f = getattr(L, f"xss_gpu_partial_qsort_{kt}")
             ms = timed(torch, lambda: wk.copy_(ks),
                        lambda: f(pk, ctypes.c_size_t(k), ctypes.c_size_t(n), 0, 0, stream(torch)), reps)
-            emit("c3", "partial_qsort", kt, n, ms, alg_bytes("select", kb), k=k,
+            emit("c3", "partial_qsort", kt, n, ms, alg_bytes("select", kb), path="select", k=k,
                  note="bytes exclude the sort of the k prefix")
         del wk, wv
     del arrays
@@ -250,7 +288,7 @@ def c3(torch, L, reps):
                lambda: L.xss_gpu_object_qsort_xyz_norm(ctypes.c_void_p(w.data_ptr()), ctypes.c_size_t(n),
                                                        ctypes.c_void_p(keys.data_ptr()), stream(torch)), reps)
     obj = 24 + 8 + alg_bytes("argsort", 8) + 48
-    emit("c3", "object_qsort", "xyz_f64_norm", n, ms, obj)
+    emit("c3", "object_qsort", "xyz_f64_norm", n, ms, obj, path="object")
 
 
 def c6(torch, L, reps):
@@ -268,7 +306,7 @@ def c6(torch, L, reps):
         for name, fn in (("xss::sort", lambda: 0 if torch.ops.xss.sort(x, False) else 1),
                          ("torch.sort", lambda: 0 if torch.sort(x, dim=-1) else 1)):
             ms = timed(torch, lambda: None, fn, reps)
-            emit("c6", name, "f32", n, ms, 2 * 4 + 2 * 8, shape="x".join(map(str, shape)))
+            emit("c6", name, "f32", n, ms, 2 * 4 + 2 * 8, path="torch", shape="x".join(map(str, shape)))
         del x
         torch.cuda.empty_cache()
 

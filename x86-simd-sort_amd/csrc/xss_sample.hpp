@@ -371,9 +371,10 @@ __device__ __forceinline__ uint64_t match_any4(uint32_t d)
 // free), which hold about one key each for spread data.  Every key's final
 // place is then its sub-bucket's start plus the number of keys of the
 // sub-bucket that order before it (by order key, ties by slot -- argsort: by
-// original index, so equal keys keep the input order); an inverse map in LDS
-// (the slot that ends at each place) then lets the workgroup write the
-// bucket out in place order, so the stores are contiguous.
+// original index, so equal keys keep the input order).  Keys only: an
+// inverse map in LDS (the slot that ends at each place) then lets the
+// workgroup write the bucket out in place order, so the stores are
+// contiguous; argsort stores each index at its place directly.
 //
 // A sub-bucket of more than SS_SUB_MAX keys (duplicates, or keys crowded in
 // a small part of the range) would make the comparisons quadratic; the
@@ -401,7 +402,10 @@ void ss_local_kernel(SsLocalArgs a)
     __shared__ U s_keys[CAP];
     __shared__ V s_vals[IDX ? CAP : 1];
     __shared__ uint32_t s_cnt[2048];  // 4096 16-bit counters / sub-bucket starts
-    __shared__ uint16_t s_inv[CAP];   // rank path: the slot whose key ends at each position
+    // rank path, keys only: the slot whose key ends at each position (argsort
+    // stores directly: its key, index and this map would not leave room for
+    // two workgroups per CU)
+    __shared__ uint16_t s_inv[IDX ? 1 : CAP];
     __shared__ uint32_t s_wc[WAVES * 16];
     __shared__ uint32_t s_wsum[WAVES];
     __shared__ uint32_t s_ao[3];
@@ -591,14 +595,14 @@ void ss_local_kernel(SsLocalArgs a)
                     }
                     r += before ? 1u : 0u;
                 }
-                s_inv[lo + r] = uint16_t(p);
+                if constexpr (IDX) put(lo + r, x, vx);
+                else s_inv[lo + r] = uint16_t(p);
             }
         }
-        __syncthreads();
-        // written in position order: consecutive threads, consecutive places
-        for (uint32_t q = tid; q < m; q += BLOCK) {
-            const uint32_t src = s_inv[q];
-            put(q, s_keys[src], IDX ? s_vals[src] : V(0));
+        if constexpr (!IDX) {
+            __syncthreads();
+            // written in position order: consecutive threads, consecutive places
+            for (uint32_t q = tid; q < m; q += BLOCK) put(q, s_keys[s_inv[q]], V(0));
         }
         return;
     }

@@ -35,7 +35,9 @@ The JSON line carries:
                       (four passes) for reference
   segmented           the batch form: 8192 uneven float32 arrays (~775M keys)
                       sorted by one graph-captured plan against a loop of
-                      plain per-array calls on the same GPU
+                      plain per-array calls on the same GPU; int32 argsort,
+                      select at len / 2, and uint64 sort and argsort of the
+                      same batch shape
   serving             the served workload (serve_bench.py) at 10k and 50k
                       requests/s: p50 / p99 latency, keys/s, launch groups
                       and coalesced requests of the server and of one plain
@@ -350,6 +352,32 @@ def segmented_batch(torch, xss_amd, L, tuning, reps=3, loops=True):
     p.close()
     out.update({"argsort_i32_ms": round(t_arg, 3), "argsort_i32_loop_ms": round(t_arg_loop, 3),
                 "qselect_ms": round(t_sel, 3), "qselect_loop_ms": round(t_sel_loop, 3)})
+    # the same batch shape with uint64 keys over the full range: sort and
+    # argsort (uint32 indices)
+    del work
+    torch.cuda.empty_cache()
+    keys64 = torch.randint(-2 ** 63, 2 ** 63 - 1, (n,), dtype=torch.int64, device="cuda", generator=g)
+    work = torch.empty_like(keys64)
+    p = plan("qsort", "u64")
+    q64 = lambda: p.run(work.data_ptr(), None, False)  # noqa: E731
+    work.copy_(keys64)
+    q64()
+    t_q64 = timed(q64, keys64)
+    ok64 = True
+    for i in range(0, nseg, 512):
+        a, b = int(off[i]), int(off[i + 1])
+        seg = work[a:b] ^ (-2 ** 63)  # unsigned order as signed
+        ok64 &= bool((seg[1:] >= seg[:-1]).all().item()) if b - a > 1 else True
+    p.close()
+    p = plan("argsort", "u64")
+    a64 = lambda: p.run(work.data_ptr(), idx.data_ptr(), False)  # noqa: E731
+    work.copy_(keys64)
+    a64()
+    t_a64 = timed(a64, keys64)
+    p.close()
+    out.update({"qsort_u64_ms": round(t_q64, 3), "qsort_u64_Gkeys_s": round(n / t_q64 / 1e6, 2),
+                "qsort_u64_sorted_ok": ok64, "argsort_u64_ms": round(t_a64, 3)})
+    del keys64
     # the CPU loop's sample: the first 1024 arrays as the plan sorted them
     m = int(off[1024])
     extra = {"sizes": sizes, "n_keys": n, "segmented_ms": t_graph,

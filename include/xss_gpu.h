@@ -217,23 +217,23 @@ XSS_API int xss_gpu_object_qsort_xyz_norm(double *xyz, size_t n, double *keys_ou
  *             "qselect" every segment partitioned at k = len / 2, as
  *                       qselect(seg, len / 2) leaves it (NaN bits kept): the
  *                       k-th element in place, smaller ones before, larger
- *                       ones after.  4-byte keys: segments above one LDS
- *                       workgroup are split by a sample sort and only the
- *                       bucket holding rank len / 2 is sorted; other widths
- *                       sort every segment (valid for every k at once)
+ *                       ones after.  4- and 8-byte keys: segments above
+ *                       one LDS workgroup are split by a sample sort and
+ *                       only the bucket holding rank len / 2 is sorted;
+ *                       2-byte keys sort every segment (valid for every k)
  *   key_type  "u16" "i16" "f16" "u32" "i32" "f32" "u64" "i64" "f64"
  *   small_max segments up to this length use one LDS workgroup each
  *             (0 = the largest the type allows); range_tiles: downsweep
  *             tiles per workgroup in larger segments (0 = 8).
- * 4-byte keys: segments up to 512 x items keys are sorted by one LDS radix
- * workgroup each; longer ones by a sample sort (splitters from a sorted
+ * 4- and 8-byte keys: segments up to 512 x items keys (8-byte keys: 4096)
+ * are sorted by one LDS workgroup each; longer ones by a sample sort (splitters from a sorted
  * sample of the segment, one histogram read and one scatter into buckets,
  * every bucket sorted in LDS).  Plan knobs, read at creation
  * (xss_gpu_set_tuning, 0 = default): "seg_bucket_keys" average bucket
  * (default 2/3 of the LDS capacity), "seg_oversample" samples per bucket
- * (32), "seg_local_items" 12 or 15 keys per LDS thread (12),
- * "seg_wide_tiles" 1024-thread scatter tiles (0), "seg_sample_off" 1 = the
- * LSD plan for 4-byte keys too.
+ * (32), "seg_local_items" keys per LDS thread, 12 or 15 for 4-byte keys (12),
+ * 8 or 12 for 8-byte keys (12; argsort: 8 or 11, default 11), "seg_wide_tiles" 1024-thread scatter tiles (0),
+ * "seg_sample_off" 1 = the LSD plan (as 2-byte keys take) for every width.
  */
 typedef struct xss_segplan *xss_segplan_t;
 XSS_API int xss_gpu_segplan_create(xss_segplan_t *plan, const char *op, const char *key_type,

@@ -1,13 +1,14 @@
-"""GPU parity of the segmented form's sample-sort plan (4-byte keys) against
-the oracle, segment by segment.
+"""GPU parity of the segmented form's sample-sort plan (4- and 8-byte keys)
+against the oracle, segment by segment.
 
 Segment lengths sit on both sides of every class boundary of the plan (LDS
-bitonic up to 4096 keys, one-workgroup LDS radix up to 512 x items, sample
-sort above), each segment with its own distribution (constant, two values,
+bitonic up to 4096 keys -- 2048 for 8-byte keys --, one-workgroup LDS sort
+up to 512 x items -- 6144 keys, 8-byte argsort 5632 --, sample sort above),
+each segment with its own distribution (constant, two values,
 duplicates, sorted, reverse, NaN, signed zeros), and the plan knobs are
 forced so that every kernel of the path runs: buckets far above the LDS
-capacity (the large-bucket passes), 15-item LDS sorts, wide scatter tiles,
-one sample per bucket."""
+capacity (the large-bucket passes), 15-item LDS sorts (8-byte keys: the
+8-item ones), wide scatter tiles, one sample per bucket."""
 import ctypes
 
 import numpy as np
@@ -18,7 +19,8 @@ from test_gpu_segmented import Plan, offsets_of
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [0, 1, 2, 3, 100, 4095, 4096, 4097, 6143, 6144, 6145, 8000, 12289, 50000, 100003, 250000, 4099, 9000]
+SIZES = [0, 1, 2, 3, 100, 4095, 4096, 4097, 6143, 6144, 6145, 8000, 12289, 50000, 100003, 250000, 4099, 9000, 2048,
+         2049, 5632, 5633]
 DISTS = ["random", "dups", "sorted", "reverse", "constant", "two", "smallrange", "nan", "signed_zeros", "rand_max"]
 KNOBS = {
     "default": {},
@@ -55,12 +57,12 @@ def knobs(xss, request):
 
 
 @pytest.mark.parametrize("knobs", list(KNOBS), indirect=True)
-@pytest.mark.parametrize("tag", ["u32", "i32", "f32"])
+@pytest.mark.parametrize("tag", ["u32", "i32", "f32", "u64", "i64", "f64"])
 @pytest.mark.parametrize("desc", [False, True])
 def test_sample_qsort(xss, knobs, tag, desc):
     O = T.ORACLE()
     keys, off = batch(tag, 31)
-    hasnan = tag == "f32"
+    hasnan = tag[0] == "f"
     d = xss.DArray.from_numpy(keys)
     Plan(xss, "qsort", tag, off, range_tiles=2).run(d, hasnan=hasnan, desc=desc)
     got = d.numpy()
@@ -71,7 +73,7 @@ def test_sample_qsort(xss, knobs, tag, desc):
 
 
 @pytest.mark.parametrize("knobs", list(KNOBS), indirect=True)
-@pytest.mark.parametrize("tag", ["u32", "i32", "f32"])
+@pytest.mark.parametrize("tag", ["u32", "i32", "f32", "u64", "i64", "f64"])
 def test_sample_argsort(xss, knobs, tag):
     """Indices are segment-local, a permutation whose gathered keys equal the
     oracle's sort; the path is stable, so equal keys keep the input order."""
@@ -93,12 +95,12 @@ def test_sample_argsort(xss, knobs, tag):
 
 
 @pytest.mark.parametrize("knobs", list(KNOBS), indirect=True)
-@pytest.mark.parametrize("tag", ["u32", "f32"])
+@pytest.mark.parametrize("tag", ["u32", "f32", "i64", "f64"])
 def test_sample_qselect(xss, knobs, tag):
     """Every segment partitioned at k = len // 2 (NaN bits kept)."""
     O = T.ORACLE()
     keys, off = batch(tag, 51)
-    hasnan = tag == "f32"
+    hasnan = tag[0] == "f"
     d = xss.DArray.from_numpy(keys)
     Plan(xss, "qselect", tag, off, range_tiles=2).run(d, hasnan=hasnan)
     got = d.numpy()
@@ -111,13 +113,50 @@ def test_sample_qselect(xss, knobs, tag):
                           f"{knobs} segment {i} len {b - a} ({DISTS[i % len(DISTS)]})")
 
 
-def test_sample_plan_info(xss):
+@pytest.mark.parametrize("tag,sizes", [("f32", [10, 5000, 6144, 6145, 100000]),
+                                       ("f64", [10, 3000, 6144, 6145, 100000])])
+def test_sample_plan_info(xss, tag, sizes):
     """The plan sorts segments above the bitonic limit and up to the LDS radix
     capacity with one workgroup each (no ranges), the rest by sample sort."""
-    sizes = np.array([10, 5000, 6144, 6145, 100000], dtype=np.int64)
-    off = offsets_of(sizes)
-    p = Plan(xss, "qsort", "f32", off)
+    off = offsets_of(np.array(sizes, dtype=np.int64))
+    p = Plan(xss, "qsort", tag, off)
     nsmall, nr, nlarge = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
     assert xss.lib().xss_gpu_segplan_info(p.p, ctypes.byref(nsmall), ctypes.byref(nr), ctypes.byref(nlarge)) == 0
     assert (nsmall.value, nlarge.value) == (1, 2)
     assert nr.value >= 2
+
+
+@pytest.fixture
+def sample_off(xss):
+    L = xss.lib()
+    assert L.xss_gpu_set_tuning(b"seg_sample_off", ctypes.c_uint64(1)) == 0
+    yield
+    L.xss_gpu_set_tuning(b"seg_sample_off", ctypes.c_uint64(0))
+
+
+@pytest.mark.parametrize("op,tag", [("qsort", "f32"), ("qsort", "u64"), ("argsort", "i64"), ("qselect", "f64")])
+def test_lsd_plan_with_sample_off(xss, sample_off, op, tag):
+    """With the sample sort switched off, 4- and 8-byte keys take the LSD
+    plan of 2-byte keys; it stays a drop-in for the same calls."""
+    O = T.ORACLE()
+    keys, off = batch(tag, 61)
+    hasnan = tag[0] == "f"
+    d = xss.DArray.from_numpy(keys)
+    p = Plan(xss, op, tag, off, range_tiles=2)
+    nsmall, nr, nlarge = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+    assert xss.lib().xss_gpu_segplan_info(p.p, ctypes.byref(nsmall), ctypes.byref(nr), ctypes.byref(nlarge)) == 0
+    idx = xss.DArray.empty("u32", keys.size) if op == "argsort" else None
+    p.run(d, idx, hasnan=hasnan)
+    got = d.numpy()
+    for i in range(len(SIZES)):
+        a, b = int(off[i]), int(off[i + 1])
+        what = f"segment {i} len {b - a} ({DISTS[i % len(DISTS)]})"
+        seg = keys[a:b]
+        if op == "qsort":
+            T.assert_sorted_equal(got[a:b], O.qsort(seg, tag, hasnan, False), tag, what)
+        elif op == "argsort":
+            gi = idx.numpy()[a:b].astype(np.uint64)
+            T.assert_argsort(gi, seg, seg[O.argsort(seg, tag).astype(np.int64)], tag, what)
+        elif b - a >= 2:
+            k = (b - a) // 2
+            T.assert_selected(got[a:b], seg, O.qselect(seg, tag, k, hasnan), k, tag, what)

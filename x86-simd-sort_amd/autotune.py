@@ -9,16 +9,18 @@ benchmark (8192 arrays, lengths log-uniform in [16, 2^20]) for every
   qsort_f32     float32 U(-1, 1) with 0.1% NaN, hasnan (the sample sort)
   argsort_i32   int32 over the full range, uint32 indices (sample sort)
   qselect_f32   the float32 arrays, k = len / 2 (sample sort)
-  qsort_u64     uint64 over the full range (the LSD plan)
+  qsort_u64     uint64 over the full range (sample sort)
+  argsort_u64   the uint64 keys, uint32 indices (sample sort)
   qsort_u16     uint16 over the full range (the LSD plan)
 
 Knobs (include/xss_gpu.h): small_max (segments up to it: one LDS bitonic
 workgroup each), range_tiles (scatter tiles per workgroup range) and, for
-4-byte keys, seg_bucket_keys (the average bucket of the sample sort: its
-bucket threshold), seg_oversample (samples per bucket), seg_local_items
-(keys per thread of the LDS sort: 12 or 15) and seg_wide_tiles (scatter
-tiles of 1024 instead of 512 threads).  The digit of every scatter is one
-byte (256 buckets or radix digits per pass).
+4- and 8-byte keys, seg_bucket_keys (the average bucket of the sample sort:
+its bucket threshold), seg_oversample (samples per bucket), seg_local_items
+(keys per thread of the LDS sort: 12 or 15; 8-byte keys 8 or 12, argsort 8
+or 11) and seg_wide_tiles (scatter tiles of 1024 instead of 512 threads).
+The digit of every scatter is one byte (256 buckets or radix digits per
+pass).
 
 Coordinate descent from the defaults: each knob in turn is swept with the
 others at the best values so far, for two rounds; every setting is timed as
@@ -41,6 +43,14 @@ SAMPLE_KNOBS = {
     "seg_bucket_keys": [2048, 3072, 4096, 5120],
     "seg_oversample": [8, 16, 32],
     "seg_local_items": [12, 15],
+    "seg_wide_tiles": [0, 1],
+}
+SAMPLE_KNOBS_W8 = {
+    "small_max": [512, 1024, 2048],
+    "range_tiles": [2, 4, 8],
+    "seg_bucket_keys": [3072, 4096, 5120],
+    "seg_oversample": [16, 32, 64],
+    "seg_local_items": [8, 12],
     "seg_wide_tiles": [0, 1],
 }
 LSD_KNOBS = {
@@ -77,7 +87,7 @@ def main():
             return v.view(torch.uint8)
         if case == "argsort_i32":
             return torch.randint(-2 ** 31, 2 ** 31, (n,), dtype=torch.int32, device="cuda", generator=g).view(torch.uint8)
-        if case == "qsort_u64":
+        if case in ("qsort_u64", "argsort_u64"):
             return torch.randint(-2 ** 63, 2 ** 63 - 1, (n,), dtype=torch.int64, device="cuda",
                                  generator=g).view(torch.uint8)
         if case == "qsort_u16":
@@ -86,7 +96,9 @@ def main():
         raise ValueError(case)
 
     cases = [("qsort_f32", "qsort", "f32", True, SAMPLE_KNOBS), ("argsort_i32", "argsort", "i32", False, SAMPLE_KNOBS),
-             ("qselect_f32", "qselect", "f32", True, SAMPLE_KNOBS), ("qsort_u64", "qsort", "u64", False, LSD_KNOBS),
+             ("qselect_f32", "qselect", "f32", True, SAMPLE_KNOBS),
+             ("qsort_u64", "qsort", "u64", False, SAMPLE_KNOBS_W8),
+             ("argsort_u64", "argsort", "u64", False, dict(SAMPLE_KNOBS_W8, seg_local_items=[8, 11])),
              ("qsort_u16", "qsort", "u16", False, LSD_KNOBS)]
     if args.only:
         keep = set(args.only.split(","))

@@ -47,14 +47,30 @@ constexpr uint32_t SS_SAMPLE_MAX = 4096;
 constexpr int SS_LOCAL_BLOCK = 512;
 constexpr int SS_BIG_BLOCK = 1024;
 constexpr int SS_BIG_ITEMS = 8;
-// the scatter's wide tile: 1024 threads x 16 keys (12 with argsort's index)
-constexpr int SS_WIDE_ITEMS_KEYS = 16;
-constexpr int SS_WIDE_ITEMS_IDX = 12;
-template <int PL>
+// the scatter's wide tile: 1024 threads x 16 items of 4 bytes, 12 of 5..8
+// bytes (a 4-byte key with argsort's index, an 8-byte key), 8 of 12 bytes
+constexpr int ss_wide_items(int item_bytes) { return item_bytes <= 4 ? 16 : item_bytes <= 8 ? 12 : 8; }
+template <class KD, int PL>
 struct SsWideShape {
     static constexpr int BLOCK = 1024;
-    static constexpr int ITEMS = PL == PL_IDX ? SS_WIDE_ITEMS_IDX : SS_WIDE_ITEMS_KEYS;
+    static constexpr int ITEMS = ss_wide_items(int(sizeof(typename KD::U)) + (PL == PL_IDX ? 4 : 0));
 };
+// keys per thread of the smaller LDS bucket sort of 8-byte keys (capacity
+// 4096; the default is 12 items, argsort 11)
+constexpr int SS_LOCAL_ITEMS_W8 = 8;
+
+// a wave-uniform value moved to scalar registers (64-bit values in halves)
+template <class U>
+__device__ __forceinline__ U uniform(U x)
+{
+    if constexpr (sizeof(U) == 8) {
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(x) >> 32));
+        return U((uint64_t(hi) << 32) | lo);
+    } else {
+        return U(__builtin_amdgcn_readfirstlane(uint32_t(x)));
+    }
+}
 
 struct SsArgs {
     // host-built plan
@@ -68,8 +84,9 @@ struct SsArgs {
     void *split;              // [nlarge][256] splitters (order keys; 255 used)
     uint32_t *bstart;         // [nbk] bucket starts (positions in the batch)
     uint32_t *bsize;          // [nbk]
-    uint32_t *brange;         // [nbk][2] lowest order key and key range of the
-                              // bucket from its splitters ([1] = 0: unknown)
+    void *brange;             // [nbk][2] of the key type: lowest order key and
+                              // key range of the bucket from its splitters
+                              // ([1] = 0: unknown)
     uint32_t *ctl;            // [0]: buckets listed in `big`
     uint32_t *big;            // [nbk] slots of the buckets larger than lcap
     uint32_t *ksel;           // [nlarge] select: the slot of the bucket holding rank len / 2
@@ -92,7 +109,7 @@ struct SplitSearch {
     __device__ __forceinline__ void load(const U *sp)
     {
 #pragma unroll
-        for (int i = 0; i < 15; ++i) sup[i] = __builtin_amdgcn_readfirstlane(sp[16 * i + 15]);
+        for (int i = 0; i < 15; ++i) sup[i] = uniform(sp[16 * i + 15]);
     }
     __device__ __forceinline__ uint32_t operator()(U o, const U *sp, uint32_t nbm1) const
     {
@@ -292,7 +309,7 @@ __global__ __launch_bounds__(BLOCK) void ss_scatter_kernel(PassArgs p, SegRanges
 // ---------------------------------------------------------------- bucket table
 
 // One workgroup per large segment, a thread per bucket.
-template <int = 0>
+template <class U>
 __global__ __launch_bounds__(256) void ss_buckets_kernel(SsArgs a, const uint32_t *segtot)
 {
     __shared__ uint32_t wsum[4];
@@ -316,10 +333,11 @@ __global__ __launch_bounds__(256) void ss_buckets_kernel(SsArgs a, const uint32_
     a.bsize[slot] = size;
     // a middle bucket holds order keys in [sp[d - 1], sp[d]); the first and
     // last are open-ended (their LDS sort measures min and max)
-    const uint32_t *sp = static_cast<const uint32_t *>(a.split) + size_t(j) * SS_SPLIT;
+    const U *sp = static_cast<const U *>(a.split) + size_t(j) * SS_SPLIT;
     const bool mid = d > 0 && d + 1 < nb && sp[d] > sp[d - 1];
-    a.brange[2 * size_t(slot)] = mid ? sp[d - 1] : 0u;
-    a.brange[2 * size_t(slot) + 1] = mid ? sp[d] - 1u - sp[d - 1] : 0u;
+    U *br = static_cast<U *>(a.brange) + 2 * size_t(slot);
+    br[0] = mid ? sp[d - 1] : U(0);
+    br[1] = mid ? U(sp[d] - 1 - sp[d - 1]) : U(0);
     const uint64_t kth = len / 2;
     const bool holds_k = pre <= kth && kth < uint64_t(pre) + size;
     if (a.select && holds_k) a.ksel[j] = slot;
@@ -332,7 +350,7 @@ struct SsLocalArgs {
     const uint32_t *slots;    // bucket mode: the slots to sort (null: slot = workgroup)
     const uint32_t *bstart;
     const uint32_t *bsize;
-    const uint32_t *brange;   // [nbk][2] key range from the splitters ([1] = 0: unknown)
+    const void *brange;       // [nbk][2] key range from the splitters ([1] = 0: unknown)
     const uint32_t *bk_seg;
     const uint32_t *lseg;
     const uint64_t *seg_off;
@@ -361,7 +379,8 @@ __device__ __forceinline__ uint64_t match_any4(uint32_t d)
 }
 
 // One workgroup sorts a bucket (or a whole medium segment) of at most
-// 512 * ITEMS keys by its order key in LDS, and writes it out.  Keys are
+// 512 * ITEMS keys (4-byte keys: 12 or 15 items, 8-byte keys: 8, 11 or 12)
+// by its order key in LDS, and writes it out.  Keys are
 // ranked by r = ord - min (min and max of the bucket): only the bits of
 // max - min are sorted.
 //
@@ -392,7 +411,6 @@ void ss_local_kernel(SsLocalArgs a)
 {
     using U = typename KD::U;
     using V = uint32_t;
-    static_assert(sizeof(U) == 4, "4-byte keys");
     constexpr int BLOCK = SS_LOCAL_BLOCK;
     constexpr int WAVES = BLOCK / 64;
     constexpr uint32_t CAP = uint32_t(BLOCK) * ITEMS;
@@ -408,10 +426,12 @@ void ss_local_kernel(SsLocalArgs a)
     __shared__ uint16_t s_inv[IDX ? 1 : CAP];
     __shared__ uint32_t s_wc[WAVES * 16];
     __shared__ uint32_t s_wsum[WAVES];
-    __shared__ uint32_t s_ao[3];
+    __shared__ U s_mm[2];          // measured min, max of the order keys
+    __shared__ uint32_t s_mx;      // largest sub-bucket
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint64_t st, segb;
-    uint32_t m, known_lo = 0, known_range = 0;
+    uint32_t m;
+    U known_lo = 0, known_range = 0;
     if (a.med) {
         const uint32_t seg = a.med[blockIdx.x];
         st = a.seg_off[seg];
@@ -422,8 +442,8 @@ void ss_local_kernel(SsLocalArgs a)
         const uint32_t slot = a.slots ? a.slots[blockIdx.x] : blockIdx.x;
         st = a.bstart[slot];
         m = a.bsize[slot];
-        known_lo = a.brange[2 * size_t(slot)];
-        known_range = a.brange[2 * size_t(slot) + 1];
+        known_lo = static_cast<const U *>(a.brange)[2 * size_t(slot)];
+        known_range = static_cast<const U *>(a.brange)[2 * size_t(slot) + 1];
         if (m > CAP) return;  // listed for ss_big
         segb = IDX ? a.seg_off[a.lseg[a.bk_seg[slot]]] : 0;
     }
@@ -446,9 +466,9 @@ void ss_local_kernel(SsLocalArgs a)
     // the bucket's key range: from its splitters, or measured (min / max)
     const bool measure = known_range == 0;
     if (tid == 0) {
-        s_ao[0] = ~0u;
-        s_ao[1] = 0u;
-        s_ao[2] = 0u;
+        s_mm[0] = U(~U(0));
+        s_mm[1] = U(0);
+        s_mx = 0u;
     }
     // keys into registers, striped: wave w holds positions [w 64 it, (w + 1) 64 it)
     const uint32_t it = (m + BLOCK - 1) / BLOCK;
@@ -474,19 +494,24 @@ void ss_local_kernel(SsLocalArgs a)
             t_min = min(t_min, U(__shfl_xor(t_min, off)));
             t_max = max(t_max, U(__shfl_xor(t_max, off)));
         }
-        __syncthreads();  // s_ao initialised
+        __syncthreads();  // s_mm initialised
         if (lane == 0) {
-            atomicMin(&s_ao[0], uint32_t(t_min));
-            atomicMax(&s_ao[1], uint32_t(t_max));
+            if constexpr (sizeof(U) == 8) {
+                atomicMin(reinterpret_cast<unsigned long long *>(&s_mm[0]), (unsigned long long)t_min);
+                atomicMax(reinterpret_cast<unsigned long long *>(&s_mm[1]), (unsigned long long)t_max);
+            } else {
+                atomicMin(&s_mm[0], t_min);
+                atomicMax(&s_mm[1], t_max);
+            }
         }
         __syncthreads();
     }
     // keys are sorted by r = ord - min, whose range fits `bits` bits (a
     // range from the splitters may include values no key has: then the
     // bucket is sorted as if keys were spread over all of it)
-    const U omin = measure ? U(__builtin_amdgcn_readfirstlane(s_ao[0])) : U(known_lo);
-    const uint32_t range = measure ? __builtin_amdgcn_readfirstlane(s_ao[1]) - uint32_t(omin) : known_range;
-    auto rel = [&](U x) { return uint32_t(KD::ord(x, flip, NL) - omin); };
+    const U omin = measure ? uniform(s_mm[0]) : known_lo;
+    const U range = measure ? U(uniform(s_mm[1]) - omin) : known_range;
+    auto rel = [&](U x) { return U(KD::ord(x, flip, NL) - omin); };
     if (range == 0) {  // all keys equal: in input order
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
@@ -497,7 +522,7 @@ void ss_local_kernel(SsLocalArgs a)
     }
     auto start_of = [&](uint32_t d) { return (s_cnt[d >> 1] >> ((d & 1u) * 16u)) & 0xffffu; };
     // 12-bit split at `sh` of the keys in registers (order inside a digit
-    // free): counts, their scan (largest count to s_ao[2]); the scatter is
+    // free): counts, their scan (largest count to s_mx); the scatter is
     // separate
     uint32_t rd[ITEMS];
     auto rank12 = [&](int sh) {
@@ -508,7 +533,7 @@ void ss_local_kernel(SsLocalArgs a)
             if (uint32_t(i) < it) {
                 const uint32_t idx = lbase + i * 64;
                 const bool real = idx < m;
-                const uint32_t d = (rel(k[i]) >> sh) & 0xfffu;
+                const uint32_t d = uint32_t(rel(k[i]) >> sh) & 0xfffu;
                 const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
                 const bool uni = __ballot(!real || d != d0) == 0;
                 uint32_t old = 0;
@@ -542,7 +567,7 @@ void ss_local_kernel(SsLocalArgs a)
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) mx = max(mx, uint32_t(__shfl_xor(mx, off)));
         if (lane == 63) s_wsum[wave] = incl;
-        if (lane == 0) atomicMax(&s_ao[2], mx);
+        if (lane == 0) atomicMax(&s_mx, mx);
         __syncthreads();
         uint32_t run = incl - sum;
         for (uint32_t w = 0; w < wave; ++w) run += s_wsum[w];
@@ -569,10 +594,10 @@ void ss_local_kernel(SsLocalArgs a)
         }
         __syncthreads();
     };
-    const int bits = 32 - __builtin_clz(range);
+    const int bits = sizeof(U) == 8 ? 64 - __builtin_clzll(uint64_t(range)) : 32 - __builtin_clz(uint32_t(range));
     const int msh = bits > 12 ? bits - 12 : 0;  // the digit (ord - min) >> msh spans [0, 4096)
     rank12(msh);
-    if (__builtin_amdgcn_readfirstlane(s_ao[2]) <= SS_SUB_MAX) {
+    if (__builtin_amdgcn_readfirstlane(s_mx) <= SS_SUB_MAX) {
         scatter12();
         // one item at a time (no register arrays here): keeps the loads of
         // different items from being hoisted together
@@ -583,7 +608,7 @@ void ss_local_kernel(SsLocalArgs a)
                 const U x = s_keys[p];
                 const V vx = IDX ? s_vals[p] : V(0);
                 const U ox = KD::ord(x, flip, NL);
-                const uint32_t d = uint32_t(ox - omin) >> msh;
+                const uint32_t d = uint32_t(U(ox - omin) >> msh);
                 const uint32_t lo = start_of(d), hi = d < 4095 ? start_of(d + 1) : m;
                 uint32_t r = 0;
                 for (uint32_t q = lo; q < hi; ++q) {
@@ -622,7 +647,7 @@ void ss_local_kernel(SsLocalArgs a)
                     k[i] = s_keys[idx];
                     if constexpr (IDX) v[i] = s_vals[idx];
                 }
-                const uint32_t d = (rel(k[i]) >> sh) & 0xfu;
+                const uint32_t d = uint32_t(rel(k[i]) >> sh) & 0xfu;
                 const uint64_t peers = match_any4(d) & __ballot(real);
                 uint32_t r = 0;
                 if (real) {
@@ -707,7 +732,7 @@ __global__ __launch_bounds__(SS_BIG_BLOCK) void ss_big_kernel(SsBigArgs g)
     constexpr bool IDX = PL == PL_IDX;
     __shared__ DownsweepSmem<KD, PL, BLOCK, ITEMS> sm;
     __shared__ uint32_t h[256];
-    __shared__ uint32_t s_ao[2];
+    __shared__ U s_ao[2];
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     const uint32_t nlist = g.ctl[0];
     const U flip = U(g.flip);
@@ -717,8 +742,8 @@ __global__ __launch_bounds__(SS_BIG_BLOCK) void ss_big_kernel(SsBigArgs g)
         const uint64_t segb = g.seg_off[g.lseg[g.bk_seg[slot]]];
         __syncthreads();  // the previous bucket's LDS readers are done
         if (tid == 0) {
-            s_ao[0] = ~0u;
-            s_ao[1] = 0u;
+            s_ao[0] = U(~U(0));
+            s_ao[1] = U(0);
         }
         __syncthreads();
         const U *yk = static_cast<const U *>(g.y_keys);
@@ -734,14 +759,19 @@ __global__ __launch_bounds__(SS_BIG_BLOCK) void ss_big_kernel(SsBigArgs g)
             t_or |= __shfl_xor(t_or, off);
         }
         if (lane == 0) {
-            atomicAnd(&s_ao[0], uint32_t(t_and));
-            atomicOr(&s_ao[1], uint32_t(t_or));
+            if constexpr (sizeof(U) == 8) {
+                atomicAnd(reinterpret_cast<unsigned long long *>(&s_ao[0]), (unsigned long long)t_and);
+                atomicOr(reinterpret_cast<unsigned long long *>(&s_ao[1]), (unsigned long long)t_or);
+            } else {
+                atomicAnd(&s_ao[0], uint32_t(t_and));
+                atomicOr(&s_ao[1], uint32_t(t_or));
+            }
         }
         __syncthreads();
-        const uint32_t diff = __builtin_amdgcn_readfirstlane(s_ao[0] ^ s_ao[1]);
+        const U diff = uniform(U(s_ao[0] ^ s_ao[1]));
         uint32_t dmask = 0;
         for (int q = 0; q < KD::PASSES; ++q)
-            if ((diff >> (8 * q)) & 0xffu) dmask |= 1u << q;
+            if (uint32_t(diff >> (8 * q)) & 0xffu) dmask |= 1u << q;
         void *sk = g.y_keys, *sv = g.y_vals, *dk = g.z_keys, *dv = g.z_vals;
         bool in_y = true;
         for (int q = 0; q < KD::PASSES; ++q) {

@@ -6,7 +6,7 @@
 // segments by size once:
 //   * small segments (len <= small_max): one LDS bitonic workgroup each
 //     (block_sort_kernel over a segment list) -- one launch for all of them;
-//   * 4-byte keys (u32 / i32 / f32):
+//   * 4- and 8-byte keys (u32 / i32 / f32 / u64 / i64 / f64):
 //       - medium segments (up to the LDS radix sort's capacity): one
 //         workgroup each sorts the segment by its order key in LDS
 //         (ss_local_kernel, xss_sample.hpp);
@@ -14,7 +14,7 @@
 //         sorted sample, one histogram read and one scatter into buckets of
 //         about `bucket_keys` keys, then every bucket sorted in LDS; qselect
 //         sorts only the bucket holding rank len / 2;
-//   * other key widths, larger segments: cut into ranges of range_tiles
+//   * 2-byte keys, larger segments: cut into ranges of range_tiles
 //     downsweep tiles; every LSD pass is upsweep (one workgroup per range) ->
 //     seg_scan_kernel (per digit, an exclusive scan over the ranges of each
 //     segment, restarting at every segment) -> seg_downsweep_kernel (one
@@ -84,12 +84,13 @@ struct xss_segplan {
     uint32_t *r_bseg = nullptr, *r_head = nullptr;
     uint32_t *rsum = nullptr, *rpre = nullptr, *segtot = nullptr;
     char *scratch = nullptr;    // ping-pong keys (+ indices)
-    // sample sort (4-byte keys)
+    // sample sort (4- and 8-byte keys)
     bool sample = false;
     uint32_t nmed = 0, nbk = 0, lcap = 0;
     uint32_t *med_list = nullptr, *lseg = nullptr, *lnb = nullptr, *lbb = nullptr, *bk_seg = nullptr;
     void *split = nullptr;
-    uint32_t *bstart = nullptr, *bsize = nullptr, *brange = nullptr, *ctl = nullptr, *big = nullptr, *ksel = nullptr;
+    uint32_t *bstart = nullptr, *bsize = nullptr, *ctl = nullptr, *big = nullptr, *ksel = nullptr;
+    void *brange = nullptr;
 };
 
 namespace {
@@ -173,7 +174,7 @@ int seg_run(xss_segplan *p, void *keys, uint32_t *idx32, bool hasnan, bool desc,
     return XSS_OK;
 }
 
-// sample-sort plan of 4-byte keys: small segments (bitonic), medium (LDS
+// sample-sort plan of 4- and 8-byte keys: small segments (bitonic), medium (LDS
 // radix, in place), large (sample sort: keys -> scratch -> keys)
 template <class KD, int PL, bool NL, int ITEMS>
 int ss_run(xss_segplan *p, void *keys, uint32_t *idx32, bool desc, hipStream_t s)
@@ -245,7 +246,7 @@ int ss_run(xss_segplan *p, void *keys, uint32_t *idx32, bool desc, hipStream_t s
         constexpr bool UN = !IDX;  // keys only: the LDS sort orders each bucket
         KernelTimer t("ss_scatter", s);
         if (p->tune.wide_tiles) {
-            using SS = SsWideShape<PL>;
+            using SS = SsWideShape<KD, PL>;
             hipLaunchKernelGGL((ss_scatter_kernel<KD, PL, SS::BLOCK, SS::ITEMS, NL, UN>), dim3(p->nr), dim3(SS::BLOCK), 0,
                                s, pa, r, a);
         } else {
@@ -257,7 +258,7 @@ int ss_run(xss_segplan *p, void *keys, uint32_t *idx32, bool desc, hipStream_t s
     XSS_HIP_CHECK(hipGetLastError());
     {
         KernelTimer t("ss_buckets", s);
-        hipLaunchKernelGGL(ss_buckets_kernel<0>, dim3(p->nbig), dim3(256), 0, s, a, static_cast<const uint32_t *>(p->segtot));
+        hipLaunchKernelGGL(ss_buckets_kernel<U>, dim3(p->nbig), dim3(256), 0, s, a, static_cast<const uint32_t *>(p->segtot));
     }
     XSS_HIP_CHECK(hipGetLastError());
     la.keys_in = yk; la.vals_in = yv;
@@ -301,18 +302,30 @@ template <class KD, int PL>
 int ss_dispatch(xss_segplan *p, void *keys, uint32_t *idx32, bool hasnan, bool desc, hipStream_t s)
 {
     const bool nl = (KD::KIND == Kind::Float) && hasnan;
-    if (p->tune.local_items == 15) {
-        if (nl) return ss_run<KD, PL, true, 15>(p, keys, idx32, desc, s);
-        return ss_run<KD, PL, false, 15>(p, keys, idx32, desc, s);
+    if constexpr (sizeof(typename KD::U) == 8) {
+        // a larger capacity for 8-byte keys: 12 items (keys only) or 11
+        // (argsort: keys and indices of two workgroups fit one CU's LDS)
+        constexpr int WIDE = PL == PL_IDX ? 11 : 12;
+        if (p->tune.local_items == WIDE) {
+            if (nl) return ss_run<KD, PL, true, WIDE>(p, keys, idx32, desc, s);
+            return ss_run<KD, PL, false, WIDE>(p, keys, idx32, desc, s);
+        }
+        if (nl) return ss_run<KD, PL, true, SS_LOCAL_ITEMS_W8>(p, keys, idx32, desc, s);
+        return ss_run<KD, PL, false, SS_LOCAL_ITEMS_W8>(p, keys, idx32, desc, s);
+    } else {
+        if (p->tune.local_items == 15) {
+            if (nl) return ss_run<KD, PL, true, 15>(p, keys, idx32, desc, s);
+            return ss_run<KD, PL, false, 15>(p, keys, idx32, desc, s);
+        }
+        if (nl) return ss_run<KD, PL, true, 12>(p, keys, idx32, desc, s);
+        return ss_run<KD, PL, false, 12>(p, keys, idx32, desc, s);
     }
-    if (nl) return ss_run<KD, PL, true, 12>(p, keys, idx32, desc, s);
-    return ss_run<KD, PL, false, 12>(p, keys, idx32, desc, s);
 }
 
 template <class KD>
 int seg_dispatch_op(xss_segplan *p, void *keys, uint32_t *idx32, bool hasnan, bool desc, hipStream_t s)
 {
-    if constexpr (sizeof(typename KD::U) == 4) {
+    if constexpr (sizeof(typename KD::U) >= 4) {
         if (p->sample) {
             if (p->op == 1) return ss_dispatch<KD, PL_IDX>(p, keys, idx32, hasnan, desc, s);
             return ss_dispatch<KD, PL_NONE>(p, keys, nullptr, hasnan, desc, s);
@@ -374,11 +387,17 @@ int xss_gpu_segplan_create(xss_segplan_t *out, const char *op, const char *key_t
     shape_of(p->tag, p->op, &maxn, &tile);
     p->tune.small_max = small_max ? std::min(small_max, maxn) : maxn;
     p->tune.range_tiles = range_tiles ? range_tiles : 8;
-    // sample sort of 4-byte keys (unless switched off): plan knobs
-    p->sample = kb == 4 && seg_knob(SEG_SAMPLE_OFF).load() == 0;
+    // sample sort of 4- and 8-byte keys (unless switched off): plan knobs
+    p->sample = kb >= 4 && seg_knob(SEG_SAMPLE_OFF).load() == 0;
     if (p->sample) {
         const uint64_t items = seg_knob(SEG_LOCAL_ITEMS).load();
-        p->tune.local_items = items == 15 ? 15 : 12;
+        if (kb == 8) {
+            const uint32_t wide = p->op == 1 ? 11 : 12;
+            p->tune.local_items = items == 8 ? SS_LOCAL_ITEMS_W8 : (items == 0 || items == wide) ? wide
+                                                                                                   : SS_LOCAL_ITEMS_W8;
+        } else {
+            p->tune.local_items = items == 15 ? 15 : 12;
+        }
         p->lcap = uint32_t(SS_LOCAL_BLOCK) * p->tune.local_items;
         const uint64_t bk = seg_knob(SEG_BUCKET_KEYS).load();
         // measured on 8192 arrays of log-uniform length: 4096-key buckets,
@@ -387,10 +406,8 @@ int xss_gpu_segplan_create(xss_segplan_t *out, const char *op, const char *key_t
         const uint64_t ov = seg_knob(SEG_OVERSAMPLE).load();
         p->tune.oversample = ov ? uint32_t(std::min<uint64_t>(ov, SS_SAMPLE_MAX)) : 32;
         p->tune.wide_tiles = seg_knob(SEG_WIDE_TILES).load() ? 1 : 0;
-        // the scatter's tile (SegShape, or HybSegShape with wide tiles)
-        const bool idx = p->op == 1;
-        tile = p->tune.wide_tiles ? 1024u * (idx ? SS_WIDE_ITEMS_IDX : SS_WIDE_ITEMS_KEYS)
-                                  : (idx ? 512u * 16u : uint32_t(XSS_SEG_BLOCK_C1) * XSS_SEG_ITEMS_C1);
+        // the scatter's tile: SsWideShape with wide tiles, else SegShape
+        if (p->tune.wide_tiles) tile = 1024u * uint32_t(ss_wide_items(kb + (p->op == 1 ? 4 : 0)));
     }
     const uint64_t rkeys = uint64_t(p->tune.range_tiles) * tile;
     // bucket the segments
@@ -454,9 +471,9 @@ int xss_gpu_segplan_create(xss_segplan_t *out, const char *op, const char *key_t
     }
     const size_t o_scr = take(scratch + 256);
     const size_t o_med = take(med.size() * 4 + 4), o_lseg = take(size_t(nbig) * 4 + 4), o_lnb = take(size_t(nbig) * 4 + 4);
-    const size_t o_lbb = take(size_t(nbig) * 4 + 4), o_bks = take(nbk * 4 + 4), o_split = take(size_t(nbig) * SS_SPLIT * 4 + 4);
+    const size_t o_lbb = take(size_t(nbig) * 4 + 4), o_bks = take(nbk * 4 + 4), o_split = take(size_t(nbig) * SS_SPLIT * kb + 4);
     const size_t o_bst = take(nbk * 4 + 4), o_bsz = take(nbk * 4 + 4), o_big = take(nbk * 4 + 4);
-    const size_t o_brg = take(nbk * 8 + 8);
+    const size_t o_brg = take(nbk * 2 * kb + 8);
     const size_t o_ksel = take(size_t(nbig) * 4 + 4), o_ctl = take(16);
     XSS_HIP_CHECK(hipGetDevice(&p->device));
     XSS_HIP_CHECK(hipMalloc(&p->mem, off));
@@ -480,7 +497,7 @@ int xss_gpu_segplan_create(xss_segplan_t *out, const char *op, const char *key_t
     p->split = m + o_split;
     p->bstart = reinterpret_cast<uint32_t *>(m + o_bst);
     p->bsize = reinterpret_cast<uint32_t *>(m + o_bsz);
-    p->brange = reinterpret_cast<uint32_t *>(m + o_brg);
+    p->brange = m + o_brg;
     p->big = reinterpret_cast<uint32_t *>(m + o_big);
     p->ksel = reinterpret_cast<uint32_t *>(m + o_ksel);
     p->ctl = reinterpret_cast<uint32_t *>(m + o_ctl);

@@ -33,6 +33,7 @@
 #include <type_traits>
 
 #include "xss_count.hpp"
+#include "xss_sample.hpp"
 #include "xss_sort.hpp"
 
 namespace xss {
@@ -898,12 +899,27 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
     }
 }
 
+// 8-byte keys, qsort and argsort: the buckets are sorted by the batch
+// form's LDS sort (ss_local_kernel: an MSD split of the bucket's measured key
+// range into 4096 sub-buckets, then a rank by comparison), which costs the
+// same whatever the keys' width, in place of six 8-bit LSD passes in LDS
+// (build-time knob; 0: hyb_local_kernel)
+#ifndef XSS_HYB_SS_LOCAL
+#define XSS_HYB_SS_LOCAL 1
+#endif
+template <class KD, int PL>
+constexpr bool hyb_ss_local()
+{
+    return XSS_HYB_SS_LOCAL && sizeof(typename KD::U) == 8 && (PL == PL_NONE || PL == PL_IDX);
+}
+constexpr uint32_t HYB_SS_CAP = uint32_t(SS_LOCAL_BLOCK) * SS_LOCAL_ITEMS_W8;
+
 template <class KD, int PL>
 struct HybPlan {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
     static constexpr bool HAS_PL = PL != PL_NONE;
-    static constexpr uint32_t CAP = HybShape<KD, PL>::CAP;
+    static constexpr uint32_t CAP = hyb_ss_local<KD, PL>() ? HYB_SS_CAP : HybShape<KD, PL>::CAP;
     // the shape (and bucket capacity) of the local sort for n keys: the
     // smallest whose capacity exceeds the average bucket by HYB_SMALL_MARGIN
     static int shape(size_t n)
@@ -915,6 +931,7 @@ struct HybPlan {
     }
     static uint32_t cap(size_t n)
     {
+        if constexpr (hyb_ss_local<KD, PL>()) return HYB_SS_CAP;
         const int sh = shape(n);
         return sh == 2 ? HybShape<KD, PL, 2>::CAP : sh == 1 ? HybShape<KD, PL, 1>::CAP : CAP;
     }
@@ -1207,14 +1224,27 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
                 hipLaunchKernelGGL((hyb_local_kernel<KD, PL, false, SZ>), dim3(HYB_BUCKETS), dim3(HYB_BLOCK), 0, s,
                                    h.bstart, h.bsize, xk, xv, idx_out, flip, 0u, plan);
         };
-        const int sh = HP::shape(n);
-        if constexpr (HybShape<KD, PL>::HAS_SMALL) {
-            if (sh == 2) launch(std::integral_constant<int, 2>{});
+        if constexpr (hyb_ss_local<KD, PL>()) {
+            SsLocalArgs la{};
+            la.bstart = h.bstart; la.bsize = h.bsize; la.keys_in = xk; la.keys_out = xk; la.vals_in = xv;
+            la.idx64_out = idx_out; la.flip = flip; la.canon = nan_last ? uint32_t(canon_nan) : 0u;
+            la.cap = HYB_SS_CAP;
+            if (nan_last)
+                hipLaunchKernelGGL((ss_local_kernel<KD, PL, true, SS_LOCAL_ITEMS_W8>), dim3(HYB_BUCKETS),
+                                   dim3(SS_LOCAL_BLOCK), 0, s, la);
+            else
+                hipLaunchKernelGGL((ss_local_kernel<KD, PL, false, SS_LOCAL_ITEMS_W8>), dim3(HYB_BUCKETS),
+                                   dim3(SS_LOCAL_BLOCK), 0, s, la);
+        } else {
+            const int sh = HP::shape(n);
+            if constexpr (HybShape<KD, PL>::HAS_SMALL) {
+                if (sh == 2) launch(std::integral_constant<int, 2>{});
+            }
+            if constexpr (HybShape<KD, PL>::HAS_MID) {
+                if (sh == 1) launch(std::integral_constant<int, 1>{});
+            }
+            if (sh == 0) launch(std::integral_constant<int, 0>{});
         }
-        if constexpr (HybShape<KD, PL>::HAS_MID) {
-            if (sh == 1) launch(std::integral_constant<int, 1>{});
-        }
-        if (sh == 0) launch(std::integral_constant<int, 0>{});
     }
     XSS_HIP_CHECK(hipGetLastError());
     {

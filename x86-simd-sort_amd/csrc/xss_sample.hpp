@@ -359,8 +359,10 @@ struct SsLocalArgs {
     const void *vals_in;      // bucket mode, argsort: the indices the scatter generated
     void *keys_out;
     uint32_t *idx32_out;      // argsort: segment-local indices
+    uint64_t *idx64_out;      // argsort of one array (non-null): 64-bit indices, no segment base
     uint64_t flip;
     uint32_t canon;           // qsort(hasnan): canonical NaN bits
+    uint32_t cap;             // bucket mode: buckets above this (0: CAP) are sorted elsewhere
 };
 
 // Peer mask of the lanes holding the same 4-bit digit (4 ballots; see
@@ -442,10 +444,12 @@ void ss_local_kernel(SsLocalArgs a)
         const uint32_t slot = a.slots ? a.slots[blockIdx.x] : blockIdx.x;
         st = a.bstart[slot];
         m = a.bsize[slot];
-        known_lo = static_cast<const U *>(a.brange)[2 * size_t(slot)];
-        known_range = static_cast<const U *>(a.brange)[2 * size_t(slot) + 1];
-        if (m > CAP) return;  // listed for ss_big
-        segb = IDX ? a.seg_off[a.lseg[a.bk_seg[slot]]] : 0;
+        if (a.brange) {
+            known_lo = static_cast<const U *>(a.brange)[2 * size_t(slot)];
+            known_range = static_cast<const U *>(a.brange)[2 * size_t(slot) + 1];
+        }
+        if (m > CAP || (a.cap && m > a.cap)) return;  // listed for the large-bucket passes
+        segb = IDX && !a.idx64_out ? a.seg_off[a.lseg[a.bk_seg[slot]]] : 0;
     }
     if (m == 0) return;
     const U flip = U(a.flip);
@@ -456,7 +460,8 @@ void ss_local_kernel(SsLocalArgs a)
     // the key (and index) that ends at bucket position q
     auto put = [&](uint32_t q, U x, V v) {
         if constexpr (IDX) {
-            iout[q] = v - uint32_t(segb);
+            if (a.idx64_out) a.idx64_out[st + q] = uint64_t(v);
+            else iout[q] = v - uint32_t(segb);
         } else {
             if constexpr (KD::KIND == Kind::Float && NL)
                 if (a.canon && KD::is_nan(x)) x = KD::QNAN;

@@ -14,7 +14,7 @@ import xss_testlib as T
 pytestmark = pytest.mark.gpu
 
 SMALL = [0, 1, 2, 3, 5, 17, 64, 100, 255, 1000, 2047, 2048, 2049, 4095, 4096]
-LARGE = [4097, 8191, 8193, 65537, 300001]
+LARGE = [4097, 6144, 6145, 8191, 8193, 65537, 300001]
 HUGE = [1 << 20, (1 << 22) + 3]
 HUGE_DISTS = ["random", "dups", "nan", "sorted"]
 

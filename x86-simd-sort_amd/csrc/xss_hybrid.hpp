@@ -1278,9 +1278,38 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     return XSS_OK;
 }
 
+// ---------------------------------------------------------------- one array in one workgroup
+//
+// qsort of an array of 4- or 8-byte keys above the LDS bitonic size and up
+// to the capacity of the batch form's LDS sort (512 x 12 keys), in place:
+// one workgroup of that sort (ss_local_kernel) in one launch, against the
+// LSD plan's 14 launches.
+constexpr int L1_ITEMS = 12;
+constexpr uint32_t L1_CAP = uint32_t(SS_LOCAL_BLOCK) * L1_ITEMS;
+
+template <class KD, bool NL>
+int enqueue_local1(void *keys, size_t n, bool desc, bool canon, hipStream_t s)
+{
+    SsLocalArgs la{};
+    la.n1 = uint32_t(n);
+    la.keys_in = keys;
+    la.keys_out = keys;
+    la.flip = desc ? ~0ull : 0ull;
+    la.canon = (canon && NL) ? 1u : 0u;
+    KernelTimer t("local1", s);
+    hipLaunchKernelGGL((ss_local_kernel<KD, PL_NONE, NL, L1_ITEMS>), dim3(1), dim3(SS_LOCAL_BLOCK), 0, s, la);
+    XSS_HIP_CHECK(hipGetLastError());
+    return XSS_OK;
+}
+
 }  // namespace xss
 
 namespace xss {
+
+// qsort of 4- and 8-byte keys up to this many keys (above the LDS bitonic
+// size, at most L1_CAP) is one LDS workgroup (xss_gpu_set_tuning
+// "local1_max_n"; 0: off)
+std::atomic<uint64_t> &local1_max_n();
 
 // arrays of at least this many keys take the MSD-first path (xss_gpu_set_tuning)
 std::atomic<uint64_t> &hybrid_min_n();
@@ -1303,6 +1332,14 @@ int run_sort_auto(const void *keys_in, void *keys_out, const void *vals_in, void
             int rc = acquire_workspace(s, cs_scratch_bytes(rows), &ws, &lock);
             if (rc) return rc;
             return enqueue_count_sort<KD>(ws->data, keys_out, n, hasnan, desc, canon, rows, s);
+        }
+    }
+    if constexpr (sizeof(typename KD::U) >= 4 && PL == PL_NONE) {
+        // arrays up to one LDS workgroup's capacity, in place
+        if (keys_in && keys_in == keys_out && !SortPlan<KD, PL>::small(n) &&
+            n <= std::min<uint64_t>(local1_max_n().load(), L1_CAP)) {
+            if ((KD::KIND == Kind::Float) && hasnan) return enqueue_local1<KD, true>(keys_out, n, desc, canon, s);
+            return enqueue_local1<KD, false>(keys_out, n, desc, canon, s);
         }
     }
     if constexpr (sizeof(typename KD::U) >= 4) {

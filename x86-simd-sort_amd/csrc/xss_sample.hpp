@@ -363,6 +363,7 @@ struct SsLocalArgs {
     uint64_t flip;
     uint32_t canon;           // qsort(hasnan): canonical NaN bits
     uint32_t cap;             // bucket mode: buckets above this (0: CAP) are sorted elsewhere
+    uint32_t n1;              // one array of n1 keys at keys_in (non-zero: no tables; keys only)
 };
 
 // Peer mask of the lanes holding the same 4-bit digit (4 ballots; see
@@ -434,7 +435,12 @@ void ss_local_kernel(SsLocalArgs a)
     uint64_t st, segb;
     uint32_t m;
     U known_lo = 0, known_range = 0;
-    if (a.med) {
+    if (a.n1) {
+        st = 0;
+        m = a.n1;
+        segb = 0;
+        if (m > CAP) return;  // the caller sends longer arrays elsewhere
+    } else if (a.med) {
         const uint32_t seg = a.med[blockIdx.x];
         st = a.seg_off[seg];
         m = uint32_t(a.seg_off[seg + 1] - st);

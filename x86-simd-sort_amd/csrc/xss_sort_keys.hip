@@ -41,6 +41,11 @@ std::atomic<uint64_t> &count_min_n()
     static std::atomic<uint64_t> v{CS_MIN_N};
     return v;
 }
+std::atomic<uint64_t> &local1_max_n()
+{
+    static std::atomic<uint64_t> v{L1_CAP};
+    return v;
+}
 // plan knobs of the segmented form (xss_segmented.hip)
 bool seg_tuning_set(const std::string &name, uint64_t value);
 }  // namespace xss
@@ -55,6 +60,10 @@ extern "C" int xss_gpu_set_tuning(const char *name, uint64_t value)
     }
     if (std::string(name) == "workspace_limit") {
         workspace_limit().store(value);
+        return XSS_OK;
+    }
+    if (std::string(name) == "local1_max_n") {
+        local1_max_n().store(value);
         return XSS_OK;
     }
     if (std::string(name) == "count_min_n") {

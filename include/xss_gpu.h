@@ -79,7 +79,7 @@ XSS_API int xss_gpu_timing_enable(int on);
  *   "count_min_n"   qsort of 16-bit keys counts keys per value (a counting
  *                   sort: one read, one write) for arrays of at least this
  *                   many keys; default 2^23.
- *   "local1_max_n"  qsort of 4- and 8-byte keys sorts an array above the
+ *   "local1_max_n"  qsort / argsort of 4- and 8-byte keys sort an array above the
  *                   LDS bitonic size (4096 keys; 2048 for 8-byte keys) and
  *                   up to this many keys (at most 6144) in one workgroup, in
  *                   one launch; default 6144, 0 = the LSD plan.

@@ -1280,24 +1280,27 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
 
 // ---------------------------------------------------------------- one array in one workgroup
 //
-// qsort of an array of 4- or 8-byte keys above the LDS bitonic size and up
-// to the capacity of the batch form's LDS sort (512 x 12 keys), in place:
-// one workgroup of that sort (ss_local_kernel) in one launch, against the
-// LSD plan's 14 launches.
+// qsort (in place) and argsort of an array of 4- or 8-byte keys above the
+// LDS bitonic size and up to the capacity of the batch form's LDS sort
+// (512 x 12 keys): one workgroup of that sort (ss_local_kernel) in one
+// launch, against the LSD plan's 14 launches.  Argsort ranks equal keys by
+// index, so its indices are the stable sort's.
 constexpr int L1_ITEMS = 12;
 constexpr uint32_t L1_CAP = uint32_t(SS_LOCAL_BLOCK) * L1_ITEMS;
 
-template <class KD, bool NL>
-int enqueue_local1(void *keys, size_t n, bool desc, bool canon, hipStream_t s)
+template <class KD, int PL, bool NL>
+int enqueue_local1(const void *keys, void *keys_out, uint64_t *idx_out, size_t n, bool desc, bool canon,
+                   hipStream_t s)
 {
     SsLocalArgs la{};
     la.n1 = uint32_t(n);
     la.keys_in = keys;
-    la.keys_out = keys;
+    la.keys_out = keys_out;
+    la.idx64_out = idx_out;  // argsort: 64-bit indices (the keys are not written)
     la.flip = desc ? ~0ull : 0ull;
-    la.canon = (canon && NL) ? 1u : 0u;
+    la.canon = (canon && NL && PL == PL_NONE) ? 1u : 0u;
     KernelTimer t("local1", s);
-    hipLaunchKernelGGL((ss_local_kernel<KD, PL_NONE, NL, L1_ITEMS>), dim3(1), dim3(SS_LOCAL_BLOCK), 0, s, la);
+    hipLaunchKernelGGL((ss_local_kernel<KD, PL, NL, L1_ITEMS>), dim3(1), dim3(SS_LOCAL_BLOCK), 0, s, la);
     XSS_HIP_CHECK(hipGetLastError());
     return XSS_OK;
 }
@@ -1334,12 +1337,13 @@ int run_sort_auto(const void *keys_in, void *keys_out, const void *vals_in, void
             return enqueue_count_sort<KD>(ws->data, keys_out, n, hasnan, desc, canon, rows, s);
         }
     }
-    if constexpr (sizeof(typename KD::U) >= 4 && PL == PL_NONE) {
-        // arrays up to one LDS workgroup's capacity, in place
-        if (keys_in && keys_in == keys_out && !SortPlan<KD, PL>::small(n) &&
-            n <= std::min<uint64_t>(local1_max_n().load(), L1_CAP)) {
-            if ((KD::KIND == Kind::Float) && hasnan) return enqueue_local1<KD, true>(keys_out, n, desc, canon, s);
-            return enqueue_local1<KD, false>(keys_out, n, desc, canon, s);
+    if constexpr (sizeof(typename KD::U) >= 4 && (PL == PL_NONE || PL == PL_IDX)) {
+        // arrays up to one LDS workgroup's capacity: qsort in place, argsort
+        const bool args_ok = keys_in && (PL == PL_IDX ? idx_out != nullptr : keys_in == keys_out);
+        if (args_ok && !SortPlan<KD, PL>::small(n) && n <= std::min<uint64_t>(local1_max_n().load(), L1_CAP)) {
+            if ((KD::KIND == Kind::Float) && hasnan)
+                return enqueue_local1<KD, PL, true>(keys_in, keys_out, idx_out, n, desc, canon, s);
+            return enqueue_local1<KD, PL, false>(keys_in, keys_out, idx_out, n, desc, canon, s);
         }
     }
     if constexpr (sizeof(typename KD::U) >= 4) {

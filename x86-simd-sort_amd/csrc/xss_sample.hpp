@@ -460,7 +460,7 @@ void ss_local_kernel(SsLocalArgs a)
     if (m == 0) return;
     const U flip = U(a.flip);
     const U *kin = static_cast<const U *>(a.keys_in) + st;
-    const V *vin = IDX && !a.med ? static_cast<const V *>(a.vals_in) + st : nullptr;
+    const V *vin = IDX && !a.med && a.vals_in ? static_cast<const V *>(a.vals_in) + st : nullptr;
     U *kout = static_cast<U *>(a.keys_out) + st;
     uint32_t *iout = a.idx32_out + st;
     // the key (and index) that ends at bucket position q

@@ -89,6 +89,8 @@ struct HybArgs {
     uint64_t *seg_andor;        // [lcap][2] AND / OR of each large bucket's raw key bits
     const uint32_t *plan;       // the device plan (hyb_plan_kernel)
     uint32_t *lrfirst;          // [lcap + 1] first range of each large bucket (large_ranges)
+    HybTable tl;                // ranges of the low passes after the first: large buckets not constant
+    uint32_t *lrfirst2;         // [lcap + 1] first range of each large bucket in tl
 };
 
 // The sort's plan, made on the device from the AND / OR of all order keys
@@ -517,6 +519,72 @@ __global__ __launch_bounds__(1024) void hyb_large_ranges_kernel(HybArgs a)
         a.td.sbase[q] = st;
         a.td.bseg[q] = lo;
         a.td.head[q] = x == st ? 1u : 0u;
+    }
+}
+
+// The ranges of the low passes after the first (one workgroup): those of
+// the large buckets whose keys are not all equal, by the AND / OR the first
+// pass's histogram read took (a bucket of equal keys stays where it is, so
+// the later passes need not walk its ranges; duplicate-heavy data has
+// thousands of them).  Segment indices stay the large buckets' own.
+template <class KD>
+__global__ __launch_bounds__(1024) void hyb_live_ranges_kernel(HybArgs a, uint32_t canon)
+{
+    using U = typename KD::U;
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t s_total;
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t nl = min(*a.nlarge, a.lcap);
+    const uint32_t per = (nl + 1023) / 1024;
+    const uint32_t c0 = min(nl, t * per), c1 = min(nl, c0 + per);
+    auto ranges_of = [&](uint32_t i) {
+        const U x = U(a.seg_andor[2 * size_t(i)]), o = U(a.seg_andor[2 * size_t(i) + 1]);
+        bool c = x == o;
+        if constexpr (KD::KIND == Kind::Float) {
+            if (c && canon && KD::is_nan(x) && x != KD::QNAN) c = false;
+        }
+        return c ? 0u : uint32_t((a.lsize[i] + a.rk_d - 1) / a.rk_d);
+    };
+    uint32_t cnt = 0;
+    for (uint32_t i = c0; i < c1; ++i) cnt += ranges_of(i);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t x = __shfl_up(incl, off);
+        if (lane >= uint32_t(off)) incl += x;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t r = incl - cnt;
+    for (uint32_t k = 0; k < w; ++k) r += wsum[k];
+    for (uint32_t i = c0; i < c1; ++i) {
+        a.lrfirst2[i] = r;
+        r += ranges_of(i);
+    }
+    if (t == 1023) {
+        const uint32_t total = min(a.tl.cap, r);
+        *a.tl.nr = total;
+        a.lrfirst2[nl] = r;
+        s_total = total;
+    }
+    __syncthreads();
+    const uint32_t total = s_total;
+    for (uint32_t q = t; q < total; q += 1024) {
+        // the bucket i with lrfirst2[i] <= q < lrfirst2[i + 1] (the last of
+        // equal starts: buckets without ranges share their successor's)
+        uint32_t lo = 0, hi = nl;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (a.lrfirst2[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        const uint64_t st = a.lstart[lo], sz = a.lsize[lo];
+        const uint64_t x = st + uint64_t(q - a.lrfirst2[lo]) * a.rk_d;
+        a.tl.begin[q] = x;
+        a.tl.end[q] = min(st + sz, x + a.rk_d);
+        a.tl.sbase[q] = st;
+        a.tl.bseg[q] = lo;
+        a.tl.head[q] = x == st ? 1u : 0u;
     }
 }
 
@@ -983,7 +1051,7 @@ struct HybPlan {
                align256(16) + table_bytes(cap_b(n)) + 2 * align256(size_t(lcap(n)) * 4) + table_bytes(cap_d(n)) +
                2 * align256(size_t(256) * cap_d(n) * 4) + align256(size_t(lcap(n)) * 256 * 4) +
                align256(HYB_CTL_BYTES) + align256(size_t(lcap(n)) * 16) + align256((size_t(lcap(n)) + 1) * 4) +
-               h2_bytes(n);
+               table_bytes(cap_d(n)) + align256((size_t(lcap(n)) + 1) * 4) + h2_bytes(n);
     }
 };
 
@@ -1050,11 +1118,16 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     h.nwctr = HYB_NWCTR;
     h.seg_andor = reinterpret_cast<uint64_t *>(take(size_t(lcap) * 16));
     h.lrfirst = reinterpret_cast<uint32_t *>(take((size_t(lcap) + 1) * 4));
+    table(h.tl, cap_d, ctr + 3);
+    h.lrfirst2 = reinterpret_cast<uint32_t *>(take((size_t(lcap) + 1) * 4));
     h.plan = plan;
     const uint32_t ranges_a = HP::ranges_a(n), rtiles_a = HP::range_tiles_a(n), spill_cap = HP::spill_cap(n);
     uint32_t *h2_res = reinterpret_cast<uint32_t *>(take(size_t(ranges_a) * 32768 * 4));
     uint32_t *h2_scnt = reinterpret_cast<uint32_t *>(take(size_t(ranges_a) * 4));
     uint32_t *h2_sbin = reinterpret_cast<uint32_t *>(take(size_t(ranges_a) * spill_cap * 4));
+    if (size_t(p - scratch) > HP::scratch_bytes(n))
+        return set_error(XSS_ERR_NOMEM, "MSD-first sort: scratch layout (%zu B) exceeds its size (%zu B)",
+                         size_t(p - scratch), HP::scratch_bytes(n));
 
     h.top_total = ws->total();
     h.segtot = segtot;
@@ -1262,10 +1335,19 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
     void *sk = xk, *sv = xv, *dk = yk, *dv = yv;
     for (int q = 0; q < P - 2; ++q) {
         // the last pass (the plan's last low digit) writes argsort's
-        // indices and canonical NaNs; which one that is is read on the device
-        rc = seg_pass(low_names, h.td, rsum_d, rpre_d, segtot_d, 8 * q, nullptr, plan + 2, uint32_t(q), sk, sv, dk,
-                      dv, false, q == 0, true, q == 0);
+        // indices and canonical NaNs; which one that is is read on the device.
+        // qsort / keyvalue_qsort: passes after the first walk only the
+        // ranges of large buckets that are not constant (argsort's last pass
+        // writes the indices of every bucket, so it keeps the whole table).
+        const bool live_only = q > 0 && PL != PL_IDX;
+        rc = seg_pass(low_names, live_only ? h.tl : h.td, rsum_d, rpre_d, segtot_d, 8 * q, nullptr, plan + 2,
+                      uint32_t(q), sk, sv, dk, dv, false, q == 0, true, q == 0);
         if (rc) return rc;
+        if (q == 0 && PL != PL_IDX && P > 3) {
+            KernelTimer t("hyb_tables", s);
+            hipLaunchKernelGGL(hyb_live_ranges_kernel<KD>, dim3(1), dim3(1024), 0, s, h, uint32_t(canon_nan));
+            XSS_HIP_CHECK(hipGetLastError());
+        }
         std::swap(sk, dk);
         std::swap(sv, dv);
     }

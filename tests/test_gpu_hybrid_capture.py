@@ -62,3 +62,46 @@ def test_captured_msd_sort_matches_eager(xss, tag, recipe, kv):
         other = xss.DArray.from_numpy(__import__("numpy").arange(5000, dtype="uint32")[::-1].copy())
         xss.qsort(other)  # eager library work between replays
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("tag,n,op", [("f32", 5000, "qsort"), ("u64", 6144, "qsort"), ("i32", 6000, "argsort"),
+                                      ("f64", 3000, "argsort")])
+def test_captured_one_workgroup_sort_matches_eager(xss, tag, n, op):
+    """Arrays above the LDS bitonic size and up to 6144 keys are sorted by
+    one workgroup in one launch, with no scratch: a captured call replays
+    to the eager result."""
+    keys = D.gen(tag, "random", n, 8)
+    s = torch.cuda.Stream()
+    work = keys.clone()
+    dk = xss.DArray(tag, n, work.view(torch.uint8))
+    idx = xss.DArray.empty("u64", n)
+
+    def run():
+        if op == "qsort":
+            xss.qsort(dk)
+        else:
+            xss.argsort(dk, out=idx)
+
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        run()
+    s.synchronize()
+    eager_k, eager_i = work.clone(), idx.numpy().copy()
+    if op == "qsort":
+        D.check_sort(eager_k, keys, tag, False, False, True, "eager")
+    graph = torch.cuda.CUDAGraph()
+    work.copy_(keys)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph, stream=s):
+        run()
+    for rep in range(3):
+        work.copy_(keys)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(work, eager_k), f"replay {rep}: keys differ from the eager run"
+        if op == "argsort":
+            assert (idx.numpy() == eager_i).all(), f"replay {rep}: indices differ from the eager run"
+        other = xss.DArray.from_numpy(__import__("numpy").arange(5000, dtype="uint32")[::-1].copy())
+        xss.qsort(other)
+        torch.cuda.synchronize()

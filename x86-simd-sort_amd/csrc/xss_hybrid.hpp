@@ -981,6 +981,18 @@ constexpr bool hyb_ss_local()
     return XSS_HYB_SS_LOCAL && sizeof(typename KD::U) == 8 && (PL == PL_NONE || PL == PL_IDX);
 }
 constexpr uint32_t HYB_SS_CAP = uint32_t(SS_LOCAL_BLOCK) * SS_LOCAL_ITEMS_W8;
+// a smaller shape (6 keys per thread, 3072-key capacity) for arrays whose
+// buckets average at most that less HYB_SMALL_MARGIN: less LDS per
+// workgroup, more workgroups per CU (build-time knob; 0: always 8 items)
+#ifndef XSS_HYB_SS_SMALL
+#define XSS_HYB_SS_SMALL 1
+#endif
+constexpr int HYB_SS_ITEMS_SMALL = 6;
+constexpr uint32_t HYB_SS_CAP_SMALL = uint32_t(SS_LOCAL_BLOCK) * HYB_SS_ITEMS_SMALL;
+inline bool hyb_ss_small(size_t n)
+{
+    return XSS_HYB_SS_SMALL && n / 65536 + 384 <= HYB_SS_CAP_SMALL;
+}
 
 template <class KD, int PL>
 struct HybPlan {
@@ -999,7 +1011,7 @@ struct HybPlan {
     }
     static uint32_t cap(size_t n)
     {
-        if constexpr (hyb_ss_local<KD, PL>()) return HYB_SS_CAP;
+        if constexpr (hyb_ss_local<KD, PL>()) return hyb_ss_small(n) ? HYB_SS_CAP_SMALL : HYB_SS_CAP;
         const int sh = shape(n);
         return sh == 2 ? HybShape<KD, PL, 2>::CAP : sh == 1 ? HybShape<KD, PL, 1>::CAP : CAP;
     }
@@ -1301,13 +1313,18 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
             SsLocalArgs la{};
             la.bstart = h.bstart; la.bsize = h.bsize; la.keys_in = xk; la.keys_out = xk; la.vals_in = xv;
             la.idx64_out = idx_out; la.flip = flip; la.canon = nan_last ? uint32_t(canon_nan) : 0u;
-            la.cap = HYB_SS_CAP;
-            if (nan_last)
-                hipLaunchKernelGGL((ss_local_kernel<KD, PL, true, SS_LOCAL_ITEMS_W8>), dim3(HYB_BUCKETS),
-                                   dim3(SS_LOCAL_BLOCK), 0, s, la);
-            else
-                hipLaunchKernelGGL((ss_local_kernel<KD, PL, false, SS_LOCAL_ITEMS_W8>), dim3(HYB_BUCKETS),
-                                   dim3(SS_LOCAL_BLOCK), 0, s, la);
+            la.cap = HP::cap(n);
+            auto go = [&](auto nl_tag, auto items_tag) {
+                hipLaunchKernelGGL((ss_local_kernel<KD, PL, decltype(nl_tag)::value, decltype(items_tag)::value>),
+                                   dim3(HYB_BUCKETS), dim3(SS_LOCAL_BLOCK), 0, s, la);
+            };
+            using SMALL = std::integral_constant<int, HYB_SS_ITEMS_SMALL>;
+            using FULL = std::integral_constant<int, SS_LOCAL_ITEMS_W8>;
+            const bool small = hyb_ss_small(n);
+            if (nan_last && small) go(std::true_type{}, SMALL{});
+            else if (nan_last) go(std::true_type{}, FULL{});
+            else if (small) go(std::false_type{}, SMALL{});
+            else go(std::false_type{}, FULL{});
         } else {
             const int sh = HP::shape(n);
             if constexpr (HybShape<KD, PL>::HAS_SMALL) {

@@ -967,7 +967,7 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
     }
 }
 
-// 8-byte keys, qsort and argsort: the buckets are sorted by the batch
+// 8-byte keys (qsort, argsort, key-value sort): the buckets are sorted by the batch
 // form's LDS sort (ss_local_kernel: an MSD split of the bucket's measured key
 // range into 4096 sub-buckets, then a rank by comparison), which costs the
 // same whatever the keys' width, in place of six 8-bit LSD passes in LDS
@@ -978,7 +978,7 @@ void hyb_local_kernel(const uint32_t *bstart, const uint32_t *bsize, void *keys_
 template <class KD, int PL>
 constexpr bool hyb_ss_local()
 {
-    return XSS_HYB_SS_LOCAL && sizeof(typename KD::U) == 8 && (PL == PL_NONE || PL == PL_IDX);
+    return XSS_HYB_SS_LOCAL && sizeof(typename KD::U) == 8;
 }
 constexpr uint32_t HYB_SS_CAP = uint32_t(SS_LOCAL_BLOCK) * SS_LOCAL_ITEMS_W8;
 // a smaller shape (6 keys per thread, 3072-key capacity) for arrays whose
@@ -993,13 +993,25 @@ inline bool hyb_ss_small(size_t n)
 {
     return XSS_HYB_SS_SMALL && n / 65536 + 384 <= HYB_SS_CAP_SMALL;
 }
+// and a larger one (12 per thread, 6144 keys) when the average bucket is
+// above 3712 keys (arrays above about 2^27.8 8-byte keys)
+constexpr int HYB_SS_ITEMS_LARGE = 12;
+constexpr uint32_t HYB_SS_CAP_LARGE = uint32_t(SS_LOCAL_BLOCK) * HYB_SS_ITEMS_LARGE;
+inline bool hyb_ss_large(size_t n)
+{
+    return n / 65536 + 384 > HYB_SS_CAP;
+}
+inline uint32_t hyb_ss_cap(size_t n)
+{
+    return hyb_ss_small(n) ? HYB_SS_CAP_SMALL : hyb_ss_large(n) ? HYB_SS_CAP_LARGE : HYB_SS_CAP;
+}
 
 template <class KD, int PL>
 struct HybPlan {
     using U = typename KD::U;
     using V = typename PayloadT<PL>::type;
     static constexpr bool HAS_PL = PL != PL_NONE;
-    static constexpr uint32_t CAP = hyb_ss_local<KD, PL>() ? HYB_SS_CAP : HybShape<KD, PL>::CAP;
+    static constexpr uint32_t CAP = hyb_ss_local<KD, PL>() ? HYB_SS_CAP_LARGE : HybShape<KD, PL>::CAP;
     // the shape (and bucket capacity) of the local sort for n keys: the
     // smallest whose capacity exceeds the average bucket by HYB_SMALL_MARGIN
     static int shape(size_t n)
@@ -1011,7 +1023,7 @@ struct HybPlan {
     }
     static uint32_t cap(size_t n)
     {
-        if constexpr (hyb_ss_local<KD, PL>()) return hyb_ss_small(n) ? HYB_SS_CAP_SMALL : HYB_SS_CAP;
+        if constexpr (hyb_ss_local<KD, PL>()) return hyb_ss_cap(n);
         const int sh = shape(n);
         return sh == 2 ? HybShape<KD, PL, 2>::CAP : sh == 1 ? HybShape<KD, PL, 1>::CAP : CAP;
     }
@@ -1312,6 +1324,7 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
         if constexpr (hyb_ss_local<KD, PL>()) {
             SsLocalArgs la{};
             la.bstart = h.bstart; la.bsize = h.bsize; la.keys_in = xk; la.keys_out = xk; la.vals_in = xv;
+            la.vals_out = xv;
             la.idx64_out = idx_out; la.flip = flip; la.canon = nan_last ? uint32_t(canon_nan) : 0u;
             la.cap = HP::cap(n);
             auto go = [&](auto nl_tag, auto items_tag) {
@@ -1319,12 +1332,18 @@ int enqueue_hybrid(Workspace *ws, char *scratch, void *keys, void *vals, uint64_
                                    dim3(HYB_BUCKETS), dim3(SS_LOCAL_BLOCK), 0, s, la);
             };
             using SMALL = std::integral_constant<int, HYB_SS_ITEMS_SMALL>;
-            using FULL = std::integral_constant<int, SS_LOCAL_ITEMS_W8>;
-            const bool small = hyb_ss_small(n);
-            if (nan_last && small) go(std::true_type{}, SMALL{});
-            else if (nan_last) go(std::true_type{}, FULL{});
-            else if (small) go(std::false_type{}, SMALL{});
-            else go(std::false_type{}, FULL{});
+            using MID = std::integral_constant<int, SS_LOCAL_ITEMS_W8>;
+            using LARGE = std::integral_constant<int, HYB_SS_ITEMS_LARGE>;
+            const int shape = hyb_ss_small(n) ? 0 : hyb_ss_large(n) ? 2 : 1;
+            if (nan_last) {
+                if (shape == 0) go(std::true_type{}, SMALL{});
+                else if (shape == 1) go(std::true_type{}, MID{});
+                else go(std::true_type{}, LARGE{});
+            } else {
+                if (shape == 0) go(std::false_type{}, SMALL{});
+                else if (shape == 1) go(std::false_type{}, MID{});
+                else go(std::false_type{}, LARGE{});
+            }
         } else {
             const int sh = HP::shape(n);
             if constexpr (HybShape<KD, PL>::HAS_SMALL) {

@@ -358,6 +358,7 @@ struct SsLocalArgs {
     const void *keys_in;      // bucket mode: the scatter's output; segment mode: the keys
     const void *vals_in;      // bucket mode, argsort: the indices the scatter generated
     void *keys_out;
+    void *vals_out;           // key-value sort: where the payloads go (with keys_out)
     uint32_t *idx32_out;      // argsort: segment-local indices
     uint64_t *idx64_out;      // argsort of one array (non-null): 64-bit indices, no segment base
     uint64_t flip;
@@ -409,19 +410,22 @@ constexpr uint32_t SS_SUB_MAX = 32;
 #define XSS_SS_LOCAL_WAVES 6
 #endif
 template <class KD, int PL, bool NL, int ITEMS>
-__global__ __launch_bounds__(SS_LOCAL_BLOCK) __attribute__((amdgpu_waves_per_eu(PL == PL_IDX ? 4 : XSS_SS_LOCAL_WAVES)))
+__global__ __launch_bounds__(SS_LOCAL_BLOCK) __attribute__((amdgpu_waves_per_eu(PL != PL_NONE ? 4 : XSS_SS_LOCAL_WAVES)))
 void ss_local_kernel(SsLocalArgs a)
 {
     using U = typename KD::U;
-    using V = uint32_t;
+    // PL_IDX: 32-bit indices (ties by index, output indices); PL_V32 /
+    // PL_V64: payloads that move with their keys (ties in any order)
+    using V = typename PayloadT<PL>::type;
     constexpr int BLOCK = SS_LOCAL_BLOCK;
     constexpr int WAVES = BLOCK / 64;
     constexpr uint32_t CAP = uint32_t(BLOCK) * ITEMS;
     constexpr bool IDX = PL == PL_IDX;
+    constexpr bool HAS_V = PL != PL_NONE;
     static_assert(CAP <= 8192, "12-bit ranks fit 13 bits, counts 16 bits");
     static_assert(WAVES * 16 <= 128, "the fallback's counters are scanned by two waves");
     __shared__ U s_keys[CAP];
-    __shared__ V s_vals[IDX ? CAP : 1];
+    __shared__ V s_vals[HAS_V ? CAP : 1];
     __shared__ uint32_t s_cnt[2048];  // 4096 16-bit counters / sub-bucket starts
     // rank path, keys only: the slot whose key ends at each position (argsort
     // stores directly: its key, index and this map would not leave room for
@@ -460,8 +464,9 @@ void ss_local_kernel(SsLocalArgs a)
     if (m == 0) return;
     const U flip = U(a.flip);
     const U *kin = static_cast<const U *>(a.keys_in) + st;
-    const V *vin = IDX && !a.med && a.vals_in ? static_cast<const V *>(a.vals_in) + st : nullptr;
+    const V *vin = HAS_V && !a.med && a.vals_in ? static_cast<const V *>(a.vals_in) + st : nullptr;
     U *kout = static_cast<U *>(a.keys_out) + st;
+    V *vout = HAS_V && !IDX ? static_cast<V *>(a.vals_out) + st : nullptr;
     uint32_t *iout = a.idx32_out + st;
     // the key (and index) that ends at bucket position q
     auto put = [&](uint32_t q, U x, V v) {
@@ -472,6 +477,7 @@ void ss_local_kernel(SsLocalArgs a)
             if constexpr (KD::KIND == Kind::Float && NL)
                 if (a.canon && KD::is_nan(x)) x = KD::QNAN;
             kout[q] = x;
+            if constexpr (HAS_V) vout[q] = v;
         }
     };
     // the bucket's key range: from its splitters, or measured (min / max)
@@ -493,6 +499,7 @@ void ss_local_kernel(SsLocalArgs a)
         const bool real = uint32_t(i) < it && idx < m;
         k[i] = real ? kin[idx] : U(0);
         if constexpr (IDX) v[i] = real ? (vin ? vin[idx] : uint32_t(st + idx)) : V(0);
+        else if constexpr (HAS_V) v[i] = real ? vin[idx] : V(0);
         if (measure && real) {
             const U o = KD::ord(k[i], flip, NL);
             t_min = min(t_min, o);
@@ -527,7 +534,7 @@ void ss_local_kernel(SsLocalArgs a)
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
             const uint32_t idx = lbase + i * 64;
-            if (uint32_t(i) < it && idx < m) put(idx, k[i], IDX ? v[i] : V(0));
+            if (uint32_t(i) < it && idx < m) put(idx, k[i], HAS_V ? v[i] : V(0));
         }
         return;
     }
@@ -600,7 +607,7 @@ void ss_local_kernel(SsLocalArgs a)
                 if (r >> 25) r = (__builtin_amdgcn_readfirstlane(r) + lane) & 0x1ffffffu;
                 const uint32_t slot = start_of(r >> 13) + (r & 0x1fffu);
                 s_keys[slot] = k[i];
-                if constexpr (IDX) s_vals[slot] = v[i];
+                if constexpr (HAS_V) s_vals[slot] = v[i];
             }
         }
         __syncthreads();
@@ -617,7 +624,7 @@ void ss_local_kernel(SsLocalArgs a)
             const uint32_t p = lbase + i * 64;
             if (p < m) {
                 const U x = s_keys[p];
-                const V vx = IDX ? s_vals[p] : V(0);
+                const V vx = HAS_V ? s_vals[p] : V(0);
                 const U ox = KD::ord(x, flip, NL);
                 const uint32_t d = uint32_t(U(ox - omin) >> msh);
                 const uint32_t lo = start_of(d), hi = d < 4095 ? start_of(d + 1) : m;
@@ -638,7 +645,10 @@ void ss_local_kernel(SsLocalArgs a)
         if constexpr (!IDX) {
             __syncthreads();
             // written in position order: consecutive threads, consecutive places
-            for (uint32_t q = tid; q < m; q += BLOCK) put(q, s_keys[s_inv[q]], V(0));
+            for (uint32_t q = tid; q < m; q += BLOCK) {
+                const uint32_t p = s_inv[q];
+                put(q, s_keys[p], HAS_V ? s_vals[p] : V(0));
+            }
         }
         return;
     }
@@ -656,7 +666,7 @@ void ss_local_kernel(SsLocalArgs a)
                 const bool real = idx < m;
                 if (from_lds && real) {
                     k[i] = s_keys[idx];
-                    if constexpr (IDX) v[i] = s_vals[idx];
+                    if constexpr (HAS_V) v[i] = s_vals[idx];
                 }
                 const uint32_t d = uint32_t(rel(k[i]) >> sh) & 0xfu;
                 const uint64_t peers = match_any4(d) & __ballot(real);
@@ -690,7 +700,7 @@ void ss_local_kernel(SsLocalArgs a)
             if (uint32_t(i) < it && lbase + i * 64 < m) {
                 const uint32_t slot = wc[rd[i] >> 16] + (rd[i] & 0xffffu);
                 s_keys[slot] = k[i];
-                if constexpr (IDX) s_vals[slot] = v[i];
+                if constexpr (HAS_V) s_vals[slot] = v[i];
             }
         }
         __syncthreads();  // the scatter and its reads of s_wc are done
@@ -707,7 +717,7 @@ void ss_local_kernel(SsLocalArgs a)
         pass4(sh, from_lds);
         from_lds = true;
     }
-    for (uint32_t p = tid; p < m; p += BLOCK) put(p, s_keys[p], IDX ? s_vals[p] : V(0));
+    for (uint32_t p = tid; p < m; p += BLOCK) put(p, s_keys[p], HAS_V ? s_vals[p] : V(0));
 }
 
 // ---------------------------------------------------------------- large buckets

@@ -83,6 +83,9 @@ XSS_API int xss_gpu_timing_enable(int on);
  *                   LDS bitonic size (4096 keys; 2048 for 8-byte keys) and
  *                   up to this many keys (at most 6144) in one workgroup, in
  *                   one launch; default 6144, 0 = the LSD plan.
+ *   "big1_max_n"    qsort of 4- and 8-byte keys above that and up to this
+ *                   many keys runs its LSD passes in one workgroup through
+ *                   the cache (two launches); default 16384, 0 = off.
  *   "workspace_limit" bytes a stream's workspace may grow to (0, the
  *                   default: no limit); a call that would need more fails
  *                   with XSS_ERR_NOMEM, like a refused allocation. */

@@ -160,3 +160,22 @@ def test_lsd_plan_with_sample_off(xss, sample_off, op, tag):
         elif b - a >= 2:
             k = (b - a) // 2
             T.assert_selected(got[a:b], seg, O.qselect(seg, tag, k, hasnan), k, tag, what)
+
+
+@pytest.mark.parametrize("tag", ["f32", "f64"])
+def test_sample_all_nan_segment_canonical(xss, tag):
+    """A segment of 20000 NaNs with assorted payloads (one bucket above the
+    LDS capacity, every order key equal) comes back as canonical NaNs."""
+    rng = np.random.default_rng(7)
+    bits = np.uint32 if tag == "f32" else np.uint64
+    exp = bits(0x7F800000) if tag == "f32" else bits(0x7FF0000000000000)
+    nan = (exp | rng.integers(1, 1 << 20, 20000).astype(bits)).view(T.TAGS[tag])
+    other = T.gen(tag, "random", 9000, 3)
+    keys = np.concatenate([nan, other]).astype(T.TAGS[tag])
+    off = offsets_of(np.array([20000, 9000], dtype=np.int64))
+    d = xss.DArray.from_numpy(keys)
+    Plan(xss, "qsort", tag, off, range_tiles=2).run(d, hasnan=True)
+    got = d.numpy()
+    assert (T.bits(got[:20000]) == T.QNAN[tag]).all()
+    O = T.ORACLE()
+    T.assert_sorted_equal(got[20000:], O.qsort(keys[20000:], tag, True, False), tag, "second segment")

@@ -78,3 +78,22 @@ def test_keyvalue_qsort(xss, ktag, vtag, desc):
         xss.keyvalue_qsort(dk, dv, hasnan, desc)
         T.assert_kv_equal(dk.numpy(), dv.numpy(), ek, ev, ktag, f"kv<{ktag},{vtag}> {dist} n={n} desc={desc}")
     del rng
+
+
+@pytest.mark.parametrize("tag,n", [("f32", 5000), ("f32", 10000), ("f64", 3000), ("f64", 10000), ("f32", 100000)])
+def test_qsort_all_nan_payloads_canonical(xss, tag, n):
+    """An array of NaNs with assorted payloads, hasnan: every key comes back
+    as the canonical NaN, also where every order key is equal and no radix
+    pass runs (the one-workgroup paths)."""
+    rng = np.random.default_rng(n)
+    u = T.TAGS[tag]
+    bits = np.uint32 if tag == "f32" else np.uint64
+    exp = np.uint32(0x7F800000) if tag == "f32" else np.uint64(0x7FF0000000000000)
+    mant = rng.integers(1, 1 << 20, n).astype(bits)
+    sign = (rng.integers(0, 2, n).astype(bits) << bits(31 if tag == "f32" else 63))
+    keys = (exp | mant | sign).view(u)
+    assert np.isnan(keys).all()
+    d = xss.DArray.from_numpy(keys)
+    xss.qsort(d, hasnan=True)
+    got = T.bits(d.numpy())
+    assert (got == T.QNAN[tag]).all(), f"{(got != T.QNAN[tag]).sum()} keys are not the canonical NaN"

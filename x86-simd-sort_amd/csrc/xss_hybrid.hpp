@@ -1423,9 +1423,60 @@ int enqueue_local1(const void *keys, void *keys_out, uint64_t *idx_out, size_t n
     return XSS_OK;
 }
 
+// qsort (in place) of an array of 4- or 8-byte keys a little above the LDS
+// sort's capacity: one 1024-thread workgroup runs the LSD passes over the
+// array in tiles through the cache (ss_big_kernel, the batch form's pass
+// for large buckets, with the array as its one bucket; passes over digits
+// every key shares are skipped), in two launches against the LSD plan's 14.
+template <int = 0>
+__global__ __launch_bounds__(64) void big1_init_kernel(uint32_t n, uint32_t *t)
+{
+    if (threadIdx.x == 0) {
+        t[0] = 1;                                  // ctl: one listed bucket
+        t[1] = 0;                                  // big[0]: slot 0
+        t[2] = 0;                                  // bstart[0]
+        t[3] = n;                                  // bsize[0]
+        t[4] = 0;                                  // bk_seg[0]
+        t[5] = 0;                                  // lseg[0]
+        reinterpret_cast<uint64_t *>(t + 6)[0] = 0;  // seg_off[0]
+    }
+}
+
+inline size_t big1_scratch_bytes(size_t n, size_t key_bytes)
+{
+    return align256(n * key_bytes) + 256;
+}
+
+template <class KD, bool NL>
+int enqueue_big1(char *scratch, void *keys, size_t n, bool desc, bool canon, hipStream_t s)
+{
+    using U = typename KD::U;
+    uint32_t *t = reinterpret_cast<uint32_t *>(scratch + align256(n * sizeof(U)));
+    {
+        KernelTimer kt("big1_init", s);
+        hipLaunchKernelGGL(big1_init_kernel<0>, dim3(1), dim3(64), 0, s, uint32_t(n), t);
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    SsBigArgs g{};
+    g.ctl = t; g.big = t + 1; g.bstart = t + 2; g.bsize = t + 3; g.bk_seg = t + 4; g.lseg = t + 5;
+    g.seg_off = reinterpret_cast<const uint64_t *>(t + 6);
+    g.y_keys = keys; g.z_keys = scratch; g.flip = desc ? ~0ull : 0ull; g.canon = (canon && NL) ? 1u : 0u;
+    g.end_in_y = 1;
+    {
+        KernelTimer kt("big1", s);
+        hipLaunchKernelGGL((ss_big_kernel<KD, PL_NONE, NL>), dim3(1), dim3(SS_BIG_BLOCK), 0, s, g);
+    }
+    XSS_HIP_CHECK(hipGetLastError());
+    return XSS_OK;
+}
+
 }  // namespace xss
 
 namespace xss {
+
+// qsort of 4- and 8-byte keys above L1_CAP and up to this many keys takes
+// the one-workgroup LSD (xss_gpu_set_tuning "big1_max_n"; 0: off)
+std::atomic<uint64_t> &big1_max_n();
 
 // qsort of 4- and 8-byte keys up to this many keys (above the LDS bitonic
 // size, at most L1_CAP) is one LDS workgroup (xss_gpu_set_tuning
@@ -1462,6 +1513,14 @@ int run_sort_auto(const void *keys_in, void *keys_out, const void *vals_in, void
             if ((KD::KIND == Kind::Float) && hasnan)
                 return enqueue_local1<KD, PL, true>(keys_in, keys_out, idx_out, n, desc, canon, s);
             return enqueue_local1<KD, PL, false>(keys_in, keys_out, idx_out, n, desc, canon, s);
+        }
+        if (PL == PL_NONE && keys_in && keys_in == keys_out && n > L1_CAP && n <= big1_max_n().load()) {
+            Workspace *ws = nullptr;
+            std::unique_lock<std::mutex> lock;
+            int rc = acquire_workspace(s, big1_scratch_bytes(n, sizeof(typename KD::U)), &ws, &lock);
+            if (rc) return rc;
+            if ((KD::KIND == Kind::Float) && hasnan) return enqueue_big1<KD, true>(ws->data, keys_out, n, desc, canon, s);
+            return enqueue_big1<KD, false>(ws->data, keys_out, n, desc, canon, s);
         }
     }
     if constexpr (sizeof(typename KD::U) >= 4) {

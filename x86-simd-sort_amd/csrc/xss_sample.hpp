@@ -838,6 +838,14 @@ __global__ __launch_bounds__(SS_BIG_BLOCK) void ss_big_kernel(SsBigArgs g)
                         if (g.canon && KD::is_nan(x)) x = KD::QNAN;
                     dst[p] = x;
                 }
+            } else if constexpr (KD::KIND == Kind::Float && NL) {
+                // no pass ran (every order key equal, e.g. all NaN): the
+                // NaNs still become the canonical NaN where they are
+                if (dmask == 0 && g.canon) {
+                    U *k = static_cast<U *>(in_y ? g.y_keys : g.z_keys);
+                    for (uint64_t p = st + tid; p < st + m; p += BLOCK)
+                        if (KD::is_nan(k[p])) k[p] = KD::QNAN;
+                }
             }
         }
     }

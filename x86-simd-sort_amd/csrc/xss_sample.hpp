@@ -367,6 +367,18 @@ struct SsLocalArgs {
     uint32_t n1;              // one array of n1 keys at keys_in (non-zero: no tables; keys only)
 };
 
+// The key whose order key is o (the inverse of KD::ord; the NaN order key
+// of hasnan gives a NaN pattern).
+template <class KD>
+__device__ __forceinline__ typename KD::U key_of_ord(typename KD::U o, typename KD::U flip)
+{
+    using U = typename KD::U;
+    const U k = U(o ^ flip);
+    if constexpr (KD::KIND == Kind::Unsigned) return k;
+    else if constexpr (KD::KIND == Kind::Signed) return U(k ^ KD::SIGN);
+    else return (k & KD::SIGN) ? U(k ^ KD::SIGN) : U(~k);
+}
+
 // Peer mask of the lanes holding the same 4-bit digit (4 ballots; see
 // match_any8).
 __device__ __forceinline__ uint64_t match_any4(uint32_t d)
@@ -598,7 +610,9 @@ void ss_local_kernel(SsLocalArgs a)
         }
         __syncthreads();
     };
-    auto scatter12 = [&]() {
+    // as_rel: store r = ord - min instead of the key (the rank compares
+    // order keys; the write-out rebuilds the key)
+    auto scatter12 = [&](bool as_rel) {
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
             if (uint32_t(i) < it) {
@@ -606,7 +620,7 @@ void ss_local_kernel(SsLocalArgs a)
                 if (r >> 26) continue;
                 if (r >> 25) r = (__builtin_amdgcn_readfirstlane(r) + lane) & 0x1ffffffu;
                 const uint32_t slot = start_of(r >> 13) + (r & 0x1fffu);
-                s_keys[slot] = k[i];
+                s_keys[slot] = as_rel ? rel(k[i]) : k[i];
                 if constexpr (HAS_V) s_vals[slot] = v[i];
             }
         }
@@ -616,7 +630,11 @@ void ss_local_kernel(SsLocalArgs a)
     const int msh = bits > 12 ? bits - 12 : 0;  // the digit (ord - min) >> msh spans [0, 4096)
     rank12(msh);
     if (__builtin_amdgcn_readfirstlane(s_mx) <= SS_SUB_MAX) {
-        scatter12();
+        // the rank compares r = ord - min, kept in LDS in place of the key
+        // when the key can be rebuilt from it (not for NaN payloads that
+        // a select or key-value sort with hasnan must keep)
+        const bool as_rel = IDX || !NL || a.canon;
+        scatter12(as_rel);
         // one item at a time (no register arrays here): keeps the loads of
         // different items from being hoisted together
 #pragma unroll 1
@@ -625,12 +643,12 @@ void ss_local_kernel(SsLocalArgs a)
             if (p < m) {
                 const U x = s_keys[p];
                 const V vx = HAS_V ? s_vals[p] : V(0);
-                const U ox = KD::ord(x, flip, NL);
-                const uint32_t d = uint32_t(U(ox - omin) >> msh);
+                const U ox = as_rel ? x : U(KD::ord(x, flip, NL) - omin);
+                const uint32_t d = uint32_t(ox >> msh);
                 const uint32_t lo = start_of(d), hi = d < 4095 ? start_of(d + 1) : m;
                 uint32_t r = 0;
                 for (uint32_t q = lo; q < hi; ++q) {
-                    const U oy = KD::ord(s_keys[q], flip, NL);
+                    const U oy = as_rel ? s_keys[q] : U(KD::ord(s_keys[q], flip, NL) - omin);
                     bool before = oy < ox;
                     if (oy == ox) {
                         if constexpr (IDX) before = s_vals[q] < vx;
@@ -647,7 +665,8 @@ void ss_local_kernel(SsLocalArgs a)
             // written in position order: consecutive threads, consecutive places
             for (uint32_t q = tid; q < m; q += BLOCK) {
                 const uint32_t p = s_inv[q];
-                put(q, s_keys[p], HAS_V ? s_vals[p] : V(0));
+                const U y = s_keys[p];
+                put(q, as_rel ? key_of_ord<KD>(U(y + omin), flip) : y, HAS_V ? s_vals[p] : V(0));
             }
         }
         return;
@@ -709,7 +728,7 @@ void ss_local_kernel(SsLocalArgs a)
     bool from_lds = false;
     if constexpr (!IDX) {
         rank12(sh);
-        scatter12();
+        scatter12(false);
         sh += 12;
         from_lds = true;
     }

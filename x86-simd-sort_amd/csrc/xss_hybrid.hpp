@@ -370,10 +370,11 @@ __global__ __launch_bounds__(1024) void hyb_msd_hist_kernel(HybTable tb, const v
 {
     using U = typename KD::U;
     __shared__ uint32_t s_off[GRP_MAX_RUNS + 1];
-    __shared__ uint32_t hist[256];
+    __shared__ uint32_t hist[4][256];  // one copy per 4 waves: fewer waves meet on a counter
     __shared__ uint32_t part[4][256];
     const uint32_t nr = *tb.nr;
     const uint32_t tid = threadIdx.x, d2 = tid & 255u, q = tid >> 8;
+    uint32_t *const hw = hist[(tid >> 6) & 3u];
     const int shift = int(dev_shift(shift_dev, 0));
     const U flip = U(flip64);
     const bool nl = nan_last != 0;
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(1024) void hyb_msd_hist_kernel(HybTable tb, const v
             if (tid == 0) s_off[ranges_a] = top_total[d1];
             loaded = d1;
         }
-        if (tid < 256) hist[tid] = 0;
+        hist[q][d2] = 0;
         __syncthreads();
         // the runs holding positions a and b - 1: the last run starting at or before them
         auto run_of = [&](uint32_t x) {
@@ -403,9 +404,28 @@ __global__ __launch_bounds__(1024) void hyb_msd_hist_kernel(HybTable tb, const v
             return lo;
         };
         const uint32_t ra = run_of(a), rb = run_of(b - 1);
+        // a wave whose keys share one digit (sorted runs) adds once, since
+        // same-address adds serialise
+        auto add = [&](U key) {
+            const uint32_t d = KD::digit(key, flip, nl, shift);
+            const uint64_t act = __ballot(1);
+            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+            if (__ballot(d != d0) == 0) {
+                if (lanes_below(act) == 0) atomicAdd(&hw[d0], uint32_t(__popcll(act)));
+            } else {
+                atomicAdd(&hw[d], 1u);
+            }
+        };
         auto count = [&](uint32_t lo, uint32_t hi) {
-            for (uint32_t x = lo + tid; x < hi; x += 1024)
-                atomicAdd(&hist[KD::digit(ykeys[sb + x], flip, nl, shift)], 1u);
+            uint32_t x = lo + tid;
+            for (; x + 3 * 1024 < hi; x += 4 * 1024) {  // four loads in flight
+                U k[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) k[j] = ykeys[sb + x + j * 1024];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) add(k[j]);
+            }
+            for (; x < hi; x += 1024) add(ykeys[sb + x]);
         };
         if (ra == rb) {
             count(a, b);
@@ -424,7 +444,9 @@ __global__ __launch_bounds__(1024) void hyb_msd_hist_kernel(HybTable tb, const v
         }
         part[q][d2] = c;
         __syncthreads();
-        if (q == 0) rsum_b[size_t(d2) * tb.cap + g] = hist[d2] + part[0][d2] + part[1][d2] + part[2][d2] + part[3][d2];
+        if (q == 0)
+            rsum_b[size_t(d2) * tb.cap + g] = hist[0][d2] + hist[1][d2] + hist[2][d2] + hist[3][d2] + part[0][d2] +
+                                              part[1][d2] + part[2][d2] + part[3][d2];
     }
 }
 

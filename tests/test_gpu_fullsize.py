@@ -3,7 +3,8 @@
 
   * qsort and argsort of every key type on every recipe of the workload
     (uniform over the full bit range, 1024 distinct values, sorted, reverse
-    sorted, and for floats 1% NaN with hasnan), ascending and descending;
+    sorted, and for floats 1% NaN with hasnan and uniform in (-1, 1)),
+    ascending and descending;
   * keyvalue_qsort of u64 Zipf(1.1) keys with their index as payload (2^27)
     and of normal float keys with u32 indices (2^28), plus 1024-distinct and
     sorted keys that take the MSD-first route; keyvalue_select at k = N/2,
@@ -32,8 +33,8 @@ import xss_testlib as T
 pytestmark = pytest.mark.gpu
 
 N_OF = {16: 1 << 29, 32: 1 << 28, 64: 1 << 27}
-RECIPES = ["random", "dups", "sorted", "reverse", "nan"]
-CELLS = [(t, r) for t in T.TAGS for r in RECIPES if r != "nan" or t in T.FLOAT_TAGS]
+RECIPES = ["random", "dups", "sorted", "reverse", "nan", "unit"]
+CELLS = [(t, r) for t in T.TAGS for r in RECIPES if r not in ("nan", "unit") or t in T.FLOAT_TAGS]
 NP_OF = {tag: np.dtype(dt) for tag, dt in T.TAGS.items()}
 
 

@@ -71,7 +71,9 @@ def is_nan(x, tag):
 def gen(tag, recipe, n, seed, device="cuda"):
     """The five full-size recipes, drawn on `device`: uniform over the full
     bit range (finite for floats), 1024 distinct values, sorted, reverse
-    sorted, and 1% canonical NaN at random positions (floats)."""
+    sorted, and 1% canonical NaN at random positions (floats); and "unit",
+    floats uniform in (-1, 1), whose skewed exponents leave buckets too
+    large for LDS after the two MSD digits."""
     nb = WIDTH[tag]
     g = torch.Generator(device=device)
     g.manual_seed(seed)
@@ -96,6 +98,10 @@ def gen(tag, recipe, n, seed, device="cuda"):
         s = torch.sort(order_key(bits(n), tag)).values
         x = from_order_key(s, tag)
         return x.flip(0) if recipe == "reverse" else x
+    if recipe == "unit":
+        ft = {16: torch.float16, 32: torch.float32, 64: torch.float64}[nb]
+        u = torch.rand(n, device=device, generator=g, dtype=torch.float64 if nb == 64 else torch.float32)
+        return (u * 2 - 1).to(ft).view(SIGNED[nb])
     if recipe == "nan":
         x = bits(n)
         if tag in FLOATS:

@@ -7,7 +7,9 @@ c1  qsort float32, N = 1,000,000, uniform in [-1e6, 1e6], ascending.
 c2  qsort and argsort, each type at 1 GiB of keys (2^28 32-bit, 2^27 64-bit,
     2^29 16-bit), five recipes -- uniform over the full bit range (finite
     floats), 1024 distinct values, sorted, reverse sorted, 1% NaN with hasnan
-    (floats) -- ascending and descending.
+    (floats) -- ascending and descending; and for the float types one more
+    recipe, "unit": uniform in (-1, 1), whose exponents are skewed (a few
+    top digits hold most keys).
 c3  keyvalue_qsort u64/u64 (2^27, Zipf s=1.1 keys below 2^32, values = index)
     and f32/u32 (2^28, normal keys); qselect / keyvalue_select at k = N/2 and
     N-1 and partial_qsort / keyvalue_partial_sort at k = 1000 and N/100 on
@@ -55,7 +57,7 @@ import xss_amd  # noqa: E402
 PEAK = 8000.0
 BYTES = {"u16": 2, "i16": 2, "f16": 2, "u32": 4, "i32": 4, "f32": 4, "u64": 8, "i64": 8, "f64": 8}
 N_FULL = {2: 1 << 29, 4: 1 << 28, 8: 1 << 27}
-RECIPES = ["random", "dups", "sorted", "reverse", "nan"]
+RECIPES = ["random", "dups", "sorted", "reverse", "nan", "unit"]
 
 
 def passes(kb):
@@ -124,6 +126,11 @@ def make_input(torch, L, tag, recipe, n, seed):
     elif recipe == "nan":
         m = torch.rand(n, device="cuda", generator=g) < 0.01
         x[m] = QNAN_INT[tag]
+    elif recipe == "unit":
+        dt = {"f16": torch.float16, "f32": torch.float32, "f64": torch.float64}[tag]
+        u = torch.rand(n, device="cuda", generator=g, dtype=torch.float64 if tag == "f64" else torch.float32)
+        x = (u * 2 - 1).to(dt).view(x.dtype)
+        del u
     torch.cuda.synchronize()
     return x
 
@@ -206,7 +213,7 @@ def c2(torch, L, reps, types=None, recipes=None):
         for recipe in RECIPES:
             if recipes and recipe not in recipes:
                 continue
-            if recipe == "nan" and tag not in ("f16", "f32", "f64"):
+            if recipe in ("nan", "unit") and tag not in ("f16", "f32", "f64"):
                 continue
             hasnan = int(recipe == "nan")
             src = make_input(torch, L, tag, recipe, n, 100 + RECIPES.index(recipe))

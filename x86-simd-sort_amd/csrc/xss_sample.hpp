@@ -632,8 +632,10 @@ void ss_local_kernel(SsLocalArgs a)
     if (__builtin_amdgcn_readfirstlane(s_mx) <= SS_SUB_MAX) {
         // the rank compares r = ord - min, kept in LDS in place of the key
         // when the key can be rebuilt from it (not for NaN payloads that
-        // a select or key-value sort with hasnan must keep)
-        const bool as_rel = IDX || !NL || a.canon;
+        // a select or key-value sort with hasnan must keep).  4-byte keys
+        // only: the 8-byte forms needed more registers (147 VGPRs for the
+        // 12-item u64 sort, 3 waves per SIMD) and ran slower
+        const bool as_rel = sizeof(U) == 4 && (IDX || !NL || a.canon);
         scatter12(as_rel);
         // one item at a time (no register arrays here): keeps the loads of
         // different items from being hoisted together
